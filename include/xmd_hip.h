@@ -1,0 +1,260 @@
+/*
+ * xmd_hip.h -- C ABI of the MI355X engine for xmd's per-step
+ * force-and-integrate loop.
+ *
+ * The reference runs this loop as OpenMP functors wired in
+ * xmd/src/freeform/main.cpp:909-1077: every force term is a pair of
+ * functors, update_X (rebuilds the term's list from the Verlet list) and
+ * eval_X_forces (accumulates F and V), and lang_pc_step integrates.
+ * This header replaces those functors with plain C entry points over
+ * caller-owned host arrays.  No C++ or torch types cross the boundary.
+ *
+ * Conventions
+ *  - Units are xmd's internal units (xmd/include/xmd/utils/units.h:9-58):
+ *    length 5 A (f77unit), time tau, energy eps, k_B = 1, charge e.
+ *  - Bead arrays are global over all replicas (replicas are concatenated);
+ *    every index that crosses the boundary is local to its replica unless
+ *    the parameter name says "global".
+ *  - Vectors are SoA: x[0..n), y[0..n), z[0..n) in one buffer of 3n.
+ *  - Errors: a status code plus xmd_last_error() (replaces the reference's
+ *    std::runtime_error / std::out_of_range, xmd/include/xmd/store/store.h:47).
+ *  - One host thread per engine; the engine owns all device memory and uses
+ *    one HIP stream.
+ */
+#ifndef XMD_HIP_H
+#define XMD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XMD_NUM_AA 20
+#define XMD_NUM_SS (XMD_NUM_AA * XMD_NUM_AA)
+/* contact_type values, xmd/include/xmd/forces/qa/contact_type.h:10-37 */
+#define XMD_QA_NUM_TYPES (4 + XMD_NUM_SS)
+/* upper bound of QA contacts one bead can hold (sum of its contact limits) */
+#define XMD_QA_MAX_SLOTS 16
+
+typedef enum xmd_status {
+    XMD_OK = 0,
+    XMD_ERR_ARG = 1,        /* bad argument / inconsistent topology       */
+    XMD_ERR_HIP = 2,        /* HIP runtime error                          */
+    XMD_ERR_CAPACITY = 3,   /* a list outgrew its device capacity         */
+    XMD_ERR_NONFINITE = 4,  /* NaN/Inf found in positions                 */
+    XMD_ERR_STATE = 5,      /* call not valid in the current engine state */
+    XMD_ERR_NO_DEVICE = 6   /* no HIP device / kernels missing            */
+} xmd_status;
+
+/* Force terms.  Bit k of a term mask enables term k; the bits correspond one
+ * to one to the reference's `enabled` flags (freeform_params sections). */
+typedef enum xmd_term {
+    XMD_TERM_TETHER = 0,    /* "tether forces"        src/forces/tether.cpp:21-36            */
+    XMD_TERM_CHIRAL = 1,    /* "chirality"            src/forces/chiral.cpp:15-37            */
+    XMD_TERM_NAT_ANG = 2,   /* "native angles"        src/forces/angle/native.cpp:13-39      */
+    XMD_TERM_CND = 3,       /* "complex native dihedrals" src/forces/dihedral/complex_native.cpp:13-49 */
+    XMD_TERM_SND = 4,       /* "simple native dihedrals"  src/forces/dihedral/simple_native.cpp:13-45  */
+    XMD_TERM_PAULI = 5,     /* "Pauli exclusion"      src/forces/pauli.cpp:32-48             */
+    XMD_TERM_GO = 6,        /* "native contacts"      src/forces/go.cpp:31-46                */
+    XMD_TERM_CONST_ES = 7,  /* "const ES"             src/forces/es/const.cpp:13-32          */
+    XMD_TERM_REL_ES = 8,    /* "relative ES"          src/forces/es/relative.cpp:13-32       */
+    XMD_TERM_QA = 9,        /* "quasi-adiabatic potential" src/forces/qa/process_contacts.cpp:13-57 */
+    XMD_TERM_PID = 10,      /* "pseudo-improper dihedral potential" src/forces/pid/eval_forces.cpp:14-141 */
+    XMD_NUM_TERMS = 11
+} xmd_term;
+
+/* Lists produced by a Verlet rebuild (xmd/include/xmd/nl/data.h:8-14 and the
+ * per-term update_X outputs), in canonical (i1 < i2) lexicographic order. */
+typedef enum xmd_list_id {
+    XMD_LIST_NL_ALL = 0,        /* every pair kept by legacy_update        */
+    XMD_LIST_NL_NATIVE = 1,     /* nl_data::native_contacts                 */
+    XMD_LIST_NL_NON_NATIVE = 2, /* nl_data::non_native                      */
+    XMD_LIST_GO = 3,            /* update_go_contacts output                */
+    XMD_LIST_PAULI = 4,         /* update_pauli_pairs output                */
+    XMD_LIST_ES = 5,            /* update_es_base output (q1*q2 != 0)       */
+    XMD_LIST_PID = 6,           /* update_pid_bundles output                */
+    XMD_LIST_QA_FREE = 7        /* QA free pairs (non-vacant, set order)    */
+} xmd_list_id;
+
+typedef enum xmd_precision {
+    XMD_PREC_FP64 = 0,   /* every force term in FP64 (the FP64 oracle path)     */
+    XMD_PREC_MIXED = 1   /* FP32 pair/bonded math from FP64-differenced
+                            displacements, FP64 sums, positions, Gear and lists */
+} xmd_precision;
+
+typedef enum xmd_lambda_version {
+    XMD_LAMBDA_COSINE = 0,    /* xmd/include/xmd/forces/pid/lambda.h:36-41 */
+    XMD_LAMBDA_ALGEBRAIC = 1  /* lambda.h:42-50                            */
+} xmd_lambda_version;
+
+/* Flat image of freeform_params (xmd/src/freeform/params.cpp:278-309) for
+ * the sections on the step path, already converted to internal units. */
+typedef struct xmd_params {
+    /* integrator + langevin (defaults.yml "integrator", "langevin") */
+    double dt;
+    double gamma_factor;
+    int32_t langevin_enabled;
+    int32_t precision;          /* xmd_precision */
+    int32_t rng_exact;          /* 1: noise bit-equal to glibc Box-Muller */
+    int32_t _pad0;
+
+    /* neighbour list ("neighbor list") */
+    double pad_factor;
+    double max_cutoff_override; /* > 0 replaces the max over term cutoffs */
+
+    /* "tether forces" */
+    int32_t tether_enabled; int32_t _pad1;
+    double tether_H1, tether_H2, tether_def_length;
+    /* "chirality" */
+    int32_t chir_enabled; int32_t _pad2;
+    double chir_e;
+    /* "native angles" */
+    int32_t nat_ang_enabled; int32_t _pad3;
+    double nat_ang_k;
+    /* "complex native dihedrals" / "simple native dihedrals" */
+    int32_t cnd_enabled; int32_t snd_enabled;
+    double cnd_CDA, cnd_CDB, snd_CDH;
+    /* "Pauli exclusion" */
+    int32_t pauli_enabled; int32_t _pad4;
+    double pauli_r_excl, pauli_depth;
+    /* "native contacts" */
+    int32_t go_enabled; int32_t _pad5;
+    double go_depth;
+    /* "const ES" / "relative ES" */
+    int32_t const_es_enabled; int32_t rel_es_enabled;
+    double const_es_screen, const_es_permittivity;
+    double rel_es_screen, rel_es_factor;
+    /* "amino acid data": charge per aa_code */
+    double aa_charge[XMD_NUM_AA];
+
+    /* "lj force variants" (lj_variants.cpp:8-63) */
+    double lj_bb_rmin, lj_bb_depth, lj_bs_rmin, lj_bs_depth;
+    double ss_rmax[XMD_NUM_SS];   /* index 20*aa1 + aa2 */
+    double ss_depth[XMD_NUM_SS];
+    int32_t ss_sinking; int32_t _pad6;
+
+    /* "pseudo-improper dihedral potential" */
+    int32_t pid_enabled; int32_t pid_lambda_version;
+    double pid_bbp_alpha, pid_bbp_psi0, pid_bbp_rmin, pid_bbp_depth;
+    double pid_bbm_alpha, pid_bbm_psi0, pid_bbm_rmin, pid_bbm_depth;
+    double pid_ss_alpha, pid_ss_psi0;
+
+    /* "quasi-adiabatic potential" */
+    int32_t qa_enabled; int32_t _pad7;
+    double qa_phase_duration, qa_breaking_factor;
+    double qa_min_cos_hr, qa_min_cos_hh, qa_max_cos_nr;
+    /* per aa: back, side_all, side_polar, side_hydrophobic (sync_data order) */
+    int8_t aa_limits[XMD_NUM_AA][4];
+    int8_t aa_polarization[XMD_NUM_AA]; /* 0 none, 1 polar, 2 hydrophobic */
+    int32_t _pad8;
+} xmd_params;
+
+/* System description.  All arrays are global over replicas, caller-owned and
+ * copied at creation.  Chains never span replicas; beads of a chain are
+ * consecutive and in sequence order. */
+typedef struct xmd_topology {
+    int32_t n_replicas;
+    int32_t n_beads;              /* total over replicas                       */
+    int32_t n_chains;
+    int32_t n_native;             /* native (Go) contacts, global indices      */
+    const int32_t *replica_offset;/* [n_replicas+1] first bead of each replica */
+    const int32_t *chain_offset;  /* [n_chains+1] first bead of each chain     */
+    const int8_t *atype;          /* [n_beads] aa_code                         */
+    const double *r0;             /* [3*n_beads] SoA initial positions          */
+    const double *box;            /* [3*n_replicas] cell lengths, 0 = open axis */
+    const double *temperature;    /* [n_replicas] eps/kB                        */
+    const uint64_t *rng_state;    /* [n_replicas] initial rand_gen state        */
+    const int32_t *nat_i1;        /* [n_native] global, i1 < i2, sorted per replica */
+    const int32_t *nat_i2;
+    const double *nat_dist;
+    /* bonded natives, indexed by the first bead of the term; NaN = absent */
+    const double *tether_len;     /* [n_beads] bond i,i+1 (NaN -> def length) */
+    const double *nat_theta;      /* [n_beads] angle i,i+1,i+2                 */
+    const double *nat_phi;        /* [n_beads] dihedral i..i+3                 */
+    const double *nat_chir;       /* [n_beads] chirality i..i+3                */
+    const double *nat_chir_factor;/* [n_beads] 1/|r23_nat|^3                   */
+} xmd_topology;
+
+/* QA contact record (xmd/include/xmd/forces/qa/contact.h:15-16). */
+typedef struct xmd_qa_contact {
+    int32_t i1, i2;       /* local indices, i1 < i2 */
+    int16_t type;         /* contact_type value      */
+    int8_t status;        /* 0 FORMING_OR_FORMED, 1 BREAKING */
+    int8_t _pad;
+    int8_t diff1[4], diff2[4];
+    double ref_time;
+} xmd_qa_contact;
+
+typedef struct xmd_engine xmd_engine;
+
+/* Engine life cycle.  Replaces the setup in main.cpp:141-907.
+ * Returns XMD_ERR_NO_DEVICE when no HIP device is visible. */
+xmd_status xmd_engine_create(const xmd_params *params, const xmd_topology *topo,
+                             xmd_engine **out);
+void xmd_engine_destroy(xmd_engine *eng);
+/* Last error message of `eng`, or of the last failed create when NULL. */
+const char *xmd_last_error(const xmd_engine *eng);
+/* Library version string. */
+const char *xmd_version(void);
+
+/* Dynamic state of one replica: Gear derivatives y0..y5 (lang_pc.h:16),
+ * y = [y0x,y0y,y0z,y1x,...,y5z] each block of n beads; rand_gen state;
+ * time (true_t).  Setting state invalidates the Verlet list (the next step
+ * or eval rebuilds it, like nl::verify's first call, verify.cpp:5-8). */
+xmd_status xmd_set_state(xmd_engine *eng, int32_t replica, const double *y,
+                         uint64_t rng_state, double t);
+/* v = y1 / dt is returned in v (3n) when non-NULL. */
+xmd_status xmd_get_state(xmd_engine *eng, int32_t replica, double *y, double *v,
+                         uint64_t *rng_state, double *t);
+/* Overwrite positions (y0) only; y1..y5 untouched.  Invalidates the list. */
+xmd_status xmd_set_positions(xmd_engine *eng, int32_t replica, const double *r);
+
+/* legacy_update + every enabled update_X (main.cpp:1063-1072) for all
+ * replicas at the current positions. */
+xmd_status xmd_rebuild_lists(xmd_engine *eng);
+/* Copy list `id` of `replica` in canonical order.  *n receives the length;
+ * at most cap entries are written (i1/i2 may be NULL to query the size). */
+xmd_status xmd_get_list(xmd_engine *eng, int32_t replica, int32_t id,
+                        int32_t *i1, int32_t *i2, int64_t cap, int64_t *n);
+
+/* Force phase only (reset_vf + eval_X + reduce_vf) for the terms in
+ * term_mask at the current positions; rebuilds the lists first if they are
+ * invalid.  QA evaluates the current contact set without changing it.
+ * F: [3*n_beads] SoA global; V_terms: [n_replicas*XMD_NUM_TERMS] or NULL. */
+xmd_status xmd_eval_forces(xmd_engine *eng, uint32_t term_mask, double *F,
+                           double *V_terms);
+
+/* QA bookkeeping of one step at the current positions and time: sift
+ * candidates, accept them in free-pair order, advance contact states
+ * (qa.cpp:20-31 order).  Forces are not returned. */
+xmd_status xmd_qa_update(xmd_engine *eng);
+/* QA state of one replica.  contacts: non-vacant in set order; sync: [4n]. */
+xmd_status xmd_get_qa_state(xmd_engine *eng, int32_t replica,
+                            xmd_qa_contact *contacts, int64_t cap,
+                            int64_t *n_contacts, int8_t *sync);
+xmd_status xmd_set_qa_state(xmd_engine *eng, int32_t replica,
+                            const xmd_qa_contact *contacts, int64_t n_contacts,
+                            const int8_t *sync);
+
+/* Run nsteps of the loop body (main.cpp:1028-1076 without reporters) on
+ * every replica: forces, Langevin predictor-corrector, verify, rebuild. */
+xmd_status xmd_step(xmd_engine *eng, int64_t nsteps);
+/* Potential energy of the last force phase per replica ([n_replicas]). */
+xmd_status xmd_get_energy(xmd_engine *eng, double *V);
+/* Counters: [0] steps taken, [1] list rebuilds summed over replicas,
+ * [2] current NL entries (full rows), [3] NL capacity, [4] QA contacts. */
+xmd_status xmd_get_counters(xmd_engine *eng, int64_t *out5);
+
+/* rand_gen noise of lang_pc (lang_pc.cpp:15-16) for n consecutive beads of
+ * one stream starting at `state`: out[3*k..3*k+2] = (normal_x2, normal) of
+ * bead k; *state_out = state after 4n draws.  Runs on the device. */
+xmd_status xmd_rand_normals(xmd_engine *eng, uint64_t state, int64_t n,
+                            float *out, uint64_t *state_out);
+
+/* Stream used by every launch of this engine (hipStream_t as void*). */
+void *xmd_stream(xmd_engine *eng);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+/*
+ * xmd_oracle.h -- CPU oracle for the per-step force-and-integrate loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++ restatement of the
+ * reference's arithmetic (file:line cited at every function in
+ * xmd_oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load it; the product path never does.
+ *
+ * It takes the same xmd_params / xmd_topology structs as the engine
+ * (include/xmd_hip.h) and exposes the same operations, so a parity test
+ * drives both sides with identical inputs.
+ */
+#ifndef XMD_ORACLE_H
+#define XMD_ORACLE_H
+#include "../include/xmd_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_sys oracle_sys;
+
+/* real_kind: 0 -> real = double (the FP64 oracle), 1 -> real = float
+ * (the reference as shipped, xmd/include/xmd/types/scalar.h:4). */
+int oracle_create(const xmd_params *p, const xmd_topology *t, int real_kind,
+                  oracle_sys **out);
+void oracle_destroy(oracle_sys *s);
+const char *oracle_last_error(void);
+
+int oracle_set_state(oracle_sys *s, int replica, const double *y,
+                     uint64_t rng_state, double t);
+int oracle_get_state(oracle_sys *s, int replica, double *y, double *v,
+                     uint64_t *rng_state, double *t);
+int oracle_set_positions(oracle_sys *s, int replica, const double *r);
+
+int oracle_rebuild_lists(oracle_sys *s);
+int oracle_get_list(oracle_sys *s, int replica, int id, int32_t *i1,
+                    int32_t *i2, int64_t cap, int64_t *n);
+/* Force phase only; does not change state.  F global SoA [3N]. */
+int oracle_eval_forces(oracle_sys *s, uint32_t mask, double *F,
+                       double *V_terms);
+int oracle_qa_update(oracle_sys *s);
+int oracle_get_qa_state(oracle_sys *s, int replica, xmd_qa_contact *c,
+                        int64_t cap, int64_t *n, int8_t *sync);
+int oracle_set_qa_state(oracle_sys *s, int replica, const xmd_qa_contact *c,
+                        int64_t n, const int8_t *sync);
+/* nthreads == 0: deterministic serial order.  nthreads > 0: the
+ * reference's OpenMP structure (thread-private F, atomic reduce), used for
+ * timing. */
+int oracle_step(oracle_sys *s, int64_t nsteps, int nthreads);
+int oracle_get_energy(oracle_sys *s, double *V);
+int oracle_get_counters(oracle_sys *s, int64_t *out5);
+
+/* rand_gen primitives (xmd/include/xmd/random/rand_gen.h:22-58). */
+uint64_t oracle_rng_next(uint64_t *state);
+float oracle_rng_uniformf(uint64_t *state);
+void oracle_rand_normals(uint64_t state, int64_t n, float *out,
+                         uint64_t *state_out);
+/* Pure primitives for closed-form tests: out = {V, dV/dr}. */
+void oracle_prim(int kind, const double *args, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1106 @@
+// engine.hip -- host side of the MI355X engine and its C ABI
+// (include/xmd_hip.h).  Replaces the functor wiring and the OpenMP step
+// loop of xmd/src/freeform/main.cpp:141-1077 with device buffers, the
+// kernels of kernels.hpp and captured HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/xmd_hip.h"
+#include "kernels.hpp"
+
+using namespace xmd_k;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+struct Fail {
+    xmd_status st;
+    std::string msg;
+};
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw Fail{XMD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_) + " (line " +    \
+                                        std::to_string(__LINE__) + ")"};                            \
+    } while (0)
+
+// ---------------------------------------------------------------- GF(2) ---
+// rand_gen's state update (rand_gen.h:24-26) is linear over GF(2); M holds
+// the images of the 64 basis vectors (columns).
+struct Mat {
+    uint64_t col[64];
+};
+uint64_t apply(const Mat &M, uint64_t x) {
+    uint64_t y = 0;
+    for (int b = 0; b < 64; ++b)
+        if ((x >> b) & 1ull) y ^= M.col[b];
+    return y;
+}
+Mat mul(const Mat &A, const Mat &B) {
+    Mat C;
+    for (int b = 0; b < 64; ++b) C.col[b] = apply(A, B.col[b]);
+    return C;
+}
+Mat identity() {
+    Mat I;
+    for (int b = 0; b < 64; ++b) I.col[b] = 1ull << b;
+    return I;
+}
+Mat step_matrix() {
+    Mat J;
+    for (int b = 0; b < 64; ++b) J.col[b] = xmd_dev::rng_step(1ull << b);
+    return J;
+}
+Mat mpow(Mat M, uint64_t k) {
+    Mat R = identity();
+    while (k) {
+        if (k & 1) R = mul(M, R);
+        M = mul(M, M);
+        k >>= 1;
+    }
+    return R;
+}
+void rows_of(const Mat &M, uint64_t *rows) {
+    for (int r = 0; r < 64; ++r) {
+        uint64_t v = 0;
+        for (int b = 0; b < 64; ++b)
+            if ((M.col[b] >> r) & 1ull) v |= 1ull << b;
+        rows[r] = v;
+    }
+}
+
+// --------------------------------------------------- glibc fix table ---
+// One byte per reachable draw value: corrections of logf / sinf / cosf
+// (bm_math.hpp).  Built once per process with the host's glibc.
+std::mutex g_fix_mu;
+std::vector<uint8_t> g_fix;
+
+uint8_t fix_code(float mine, float ref) {
+    if (mine == ref || (mine != mine && ref != ref)) return 0;
+    if (ref == xmd_bm::next_up(mine)) return 1;
+    if (ref == xmd_bm::next_down(mine)) return 2;
+    throw Fail{XMD_ERR_STATE, "glibc libm differs from the engine's by more than one ulp"};
+}
+
+const std::vector<uint8_t> &fix_table() {
+    std::lock_guard<std::mutex> lk(g_fix_mu);
+    if (!g_fix.empty()) return g_fix;
+    const uint32_t N = xmd_bm::NUM_DRAW_VALUES;
+    std::vector<uint8_t> tab(N);
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::vector<int> bad(nt, 0);
+    const float two_pi = (float)(2.0 * M_PI);
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            try {
+                for (uint32_t k = t; k < N; k += nt) {
+                    // value with draw_index(v) == k
+                    float v;
+                    if (k < 16777216u) v = (float)k;
+                    else {
+                        uint32_t q = k - 16777216u, e = q >> 23, m = q & 0x7fffffu;
+                        v = xmd_bm::bitsf(((151u + e) << 23) | m);
+                    }
+                    float x = v * (1.0f / 4294967296.0f);
+                    uint8_t c = fix_code(xmd_bm::logf_cr(x), logf(x));
+                    float tt = two_pi * x;
+                    double sd, cd;
+                    xmd_bm::sincos_red((double)tt, sd, cd);
+                    c |= (uint8_t)(fix_code((float)sd, sinf(tt)) << 2);
+                    c |= (uint8_t)(fix_code((float)cd, cosf(tt)) << 4);
+                    tab[k] = c;
+                }
+            } catch (...) {
+                bad[t] = 1;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    for (int b : bad)
+        if (b) throw Fail{XMD_ERR_STATE, "glibc libm differs from the engine's by more than one ulp"};
+    g_fix.swap(tab);
+    return g_fix;
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+// ================================================================ engine ===
+struct xmd_engine {
+    xmd_params P;
+    int nrep = 0, nbeads = 0, nslot = 0, nchains = 0;
+    std::vector<int> rep_off, rep_slot_off, rep_n, slot_of_bead, bead_of_slot, slot_rep_h;
+    Derived G;
+    TermTab<double> TTd;
+    TermTab<float> TTf;
+    Dev D;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    int cur = 0;
+    std::string err;
+    bool pid_split = false;
+    int nfp_chunks = 0, nct_chunks = 0, qa_grid = 0;
+    uint32_t mask = 0;
+    hipGraphExec_t g_chunk[2] = {nullptr, nullptr}, g_one[2] = {nullptr, nullptr};
+    int chunk_steps = 16;
+    int64_t steps = 0;
+    std::vector<char> pending_rb;
+    std::vector<int> nl_base_h, nl_cap_h, fp_base_h, fp_cap_h, ct_base_h, ct_cap_h;
+    int nl_total = 0, fp_total = 0, ct_total = 0;
+
+    template <typename T> T *alloc(size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        HIPCHK(hipMalloc(&p, n * sizeof(T)));
+        HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), stream));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+    // Host-to-device copies are synchronous: the source vectors are
+    // temporaries and the stream may still hold earlier work on `d`.
+    template <typename T> void put(T *d, const std::vector<T> &h) {
+        if (h.empty()) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    template <typename T> T *upload(const std::vector<T> &h) {
+        T *d = alloc<T>(h.size());
+        put(d, h);
+        return d;
+    }
+    template <typename T> std::vector<T> get(const T *d, size_t n) {
+        std::vector<T> h(n);
+        if (n) HIPCHK(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return h;
+    }
+    ~xmd_engine() {
+        for (auto &g : g_chunk)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto &g : g_one)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int nblk() const { return nslot / BLK; }
+
+    // ------------------------------------------------------------ steps ---
+    void rebuild_seq(const double *Ppos, int p) {
+        int nb = nblk();
+        k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p);
+        k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0);
+        k_cell_fill<<<nb, BLK, 0, stream>>>(D, p);
+        k_nl_count<<<nb, BLK, 0, stream>>>(D, G, Ppos, p);
+        k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 1);
+        k_nl_fill<<<nb, BLK, 0, stream>>>(D, G, Ppos, p);
+        k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2);
+        k_fp_write<<<nb, BLK, 0, stream>>>(D, p);
+    }
+    template <typename T> void force_launch(const TermTab<T> &TT, int p, bool fused) {
+        int nb = nblk();
+        if (fused) k_force<T, true, false><<<nb, BLK, 0, stream>>>(D, G, TT, p);
+        else k_force<T, false, false><<<nb, BLK, 0, stream>>>(D, G, TT, p);
+    }
+    void step_seq(int p) {
+        if (G.qa_on) {
+            k_qa_pairs<<<qa_grid, BLK, 0, stream>>>(D, G, p, nfp_chunks, nct_chunks, 1);
+            k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p);
+        }
+        bool fused = !pid_split;
+        if (P.precision == XMD_PREC_MIXED) force_launch(TTf, p, fused);
+        else force_launch(TTd, p, fused);
+        if (!fused) k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p);
+        k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1);
+        rebuild_seq(D.pos[1 - p], p);
+    }
+    hipGraphExec_t capture(int p, int nsteps) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < nsteps; ++k) step_seq((p + k) & 1);
+        HIPCHK(hipStreamEndCapture(stream, &g));
+        hipGraphExec_t ex;
+        HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        return ex;
+    }
+    void ensure_lists() {
+        bool any = false;
+        for (char c : pending_rb) any |= c != 0;
+        if (!any) return;
+        std::vector<int> f(nrep);
+        for (int k = 0; k < nrep; ++k) f[k] = pending_rb[k];
+        put(D.force_rb, f);
+        k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, cur, 0);
+        rebuild_seq(D.pos[cur], cur);
+        std::fill(pending_rb.begin(), pending_rb.end(), 0);
+        check_err();
+    }
+    void check_err() {
+        auto e = get(D.err, 1);
+        if (e[0]) {
+            std::string m = "device error:";
+            if (e[0] & E_NL_CAP) m += " neighbour-list capacity exceeded;";
+            if (e[0] & E_FP_CAP) m += " QA free-pair capacity exceeded;";
+            if (e[0] & E_CT_CAP) m += " QA contact capacity exceeded;";
+            if (e[0] & E_SLOT) m += " more QA contacts on a bead than slots;";
+            if (e[0] & E_NONFINITE) throw Fail{XMD_ERR_NONFINITE, "non-finite positions"};
+            throw Fail{XMD_ERR_CAPACITY, m};
+        }
+    }
+    void run_steps(int64_t n) {
+        ensure_lists();
+        while (n > 0) {
+            if (n >= chunk_steps) {
+                if (!g_chunk[cur]) g_chunk[cur] = capture(cur, chunk_steps);
+                HIPCHK(hipGraphLaunch(g_chunk[cur], stream));
+                if (chunk_steps & 1) cur ^= 1;
+                n -= chunk_steps;
+                steps += chunk_steps;
+            } else {
+                if (!g_one[cur]) g_one[cur] = capture(cur, 1);
+                HIPCHK(hipGraphLaunch(g_one[cur], stream));
+                cur ^= 1;
+                n -= 1;
+                steps += 1;
+            }
+        }
+        check_err();
+    }
+};
+
+namespace {
+
+// Derived constants, main.cpp:345-907 expressions in double.
+void derive(xmd_engine &E, const xmd_topology *t) {
+    const xmd_params &p = E.P;
+    Derived &G = E.G;
+    G.dt = p.dt;
+    G.dt_inv = 1.0 / p.dt;
+    G.half_dt2 = p.dt * p.dt / 2.0;
+    G.gamma = p.gamma_factor * 1.0;
+    G.lang_on = p.langevin_enabled != 0;
+    double bb_rmin = p.lj_bb_rmin, bs_rmin = p.lj_bs_rmin;
+    std::vector<double> ss_rmax(p.ss_rmax, p.ss_rmax + XMD_NUM_SS);
+    double es_cut = p.const_es_enabled ? 2.0 * p.const_es_screen : 2.0 * p.rel_es_screen;
+    double pid_cut = 0.0;
+    pid_cut = std::max(pid_cut, 2.0 * bb_rmin);
+    pid_cut = std::max(pid_cut, 2.0 * bs_rmin);
+    pid_cut = std::max(pid_cut, 2.0 * bs_rmin);
+    for (double r : ss_rmax) pid_cut = std::max(pid_cut, 2.0 * r);
+    double mfmd = 0.0;
+    mfmd = std::max(mfmd, bb_rmin);
+    mfmd = std::max(mfmd, bs_rmin);
+    for (double r : ss_rmax) mfmd = std::max(mfmd, r);
+    double mc = 0.0;
+    if (p.pauli_enabled) mc = std::max(mc, p.pauli_r_excl);
+    if (p.go_enabled)
+        for (int k = 0; k < t->n_native; ++k) mc = std::max(mc, 2.0 * t->nat_dist[k]);
+    if (p.const_es_enabled || p.rel_es_enabled) mc = std::max(mc, es_cut);
+    if (p.qa_enabled) mc = std::max(mc, mfmd);
+    if (p.pid_enabled) mc = std::max(mc, pid_cut);
+    if (p.max_cutoff_override > 0) mc = p.max_cutoff_override;
+    G.pad = p.pad_factor * mc;
+    G.req_r = mc + G.pad;
+    G.pauli_min_inv = 1.0 / (p.pauli_r_excl + G.pad);
+    G.es_min_inv = 1.0 / (es_cut + G.pad);
+    G.pid_min_inv = 1.0 / (pid_cut + G.pad);
+    G.qa_min_inv = 1.0 / (mfmd + G.pad);
+    G.pauli_on = p.pauli_enabled;
+    G.es_on = p.const_es_enabled || p.rel_es_enabled;
+    G.pid_on = p.pid_enabled;
+    G.qa_on = p.qa_enabled;
+    G.go_on = p.go_enabled;
+    QaParams &Q = G.qa;
+    Q.min_cos_hr = p.qa_min_cos_hr;
+    Q.min_cos_hh = p.qa_min_cos_hh;
+    Q.max_cos_nr = p.qa_max_cos_nr;
+    Q.cycle_time = p.qa_phase_duration;
+    Q.cycle_time_inv = 1.0 / Q.cycle_time;
+    Q.factor = p.qa_breaking_factor * (double)powf(2.0f, -1.0f / 6.0f);
+    Q.bb_depth = p.lj_bb_depth;
+    Q.bb_rmin = bb_rmin;
+    Q.bs_depth = p.lj_bs_depth;
+    Q.bs_rmin = bs_rmin;
+
+    auto fill = [&](auto &TT) {
+        using T = std::remove_reference_t<decltype(TT.P.H1)>;
+        TermParams<T> &Pt = TT.P;
+        Pt.H1 = (T)p.tether_H1;
+        Pt.H2 = (T)p.tether_H2;
+        Pt.e_chi = (T)p.chir_e;
+        Pt.ang_k = (T)p.nat_ang_k;
+        Pt.CDA = (T)p.cnd_CDA;
+        Pt.CDB = (T)p.cnd_CDB;
+        Pt.CDH = (T)p.snd_CDH;
+        Pt.pauli_r_excl = (T)p.pauli_r_excl;
+        Pt.pauli_depth = (T)p.pauli_depth;
+        Pt.go_depth = (T)p.go_depth;
+        Pt.es_relative = p.const_es_enabled ? 0 : 1;
+        double Vf = p.const_es_enabled ? 1.0 / (4.0 * M_PI * p.const_es_permittivity)
+                                       : 1.0 / (4.0 * M_PI * p.rel_es_factor);
+        double sdi = p.const_es_enabled ? 1.0 / p.const_es_screen : 1.0 / p.rel_es_screen;
+        Pt.es_Vf = (T)Vf;
+        Pt.es_sdi = (T)sdi;
+        Pt.bbp_psi0 = (T)p.pid_bbp_psi0;
+        Pt.bbp_alpha = (T)p.pid_bbp_alpha;
+        Pt.bbp_rmin = (T)p.pid_bbp_rmin;
+        Pt.bbp_depth = (T)p.pid_bbp_depth;
+        Pt.bbm_psi0 = (T)p.pid_bbm_psi0;
+        Pt.bbm_alpha = (T)p.pid_bbm_alpha;
+        Pt.bbm_rmin = (T)p.pid_bbm_rmin;
+        Pt.bbm_depth = (T)p.pid_bbm_depth;
+        Pt.ss_psi0 = (T)p.pid_ss_psi0;
+        Pt.ss_alpha = (T)p.pid_ss_alpha;
+        Pt.lambda_version = p.pid_lambda_version;
+    };
+    fill(E.TTd);
+    fill(E.TTf);
+    uint32_t m = 0;
+    if (p.tether_enabled) m |= 1u << T_TETHER;
+    if (p.chir_enabled) m |= 1u << T_CHIRAL;
+    if (p.nat_ang_enabled) m |= 1u << T_NAT_ANG;
+    if (p.cnd_enabled) m |= 1u << T_CND;
+    if (p.snd_enabled) m |= 1u << T_SND;
+    if (p.pauli_enabled) m |= 1u << T_PAULI;
+    if (p.go_enabled) m |= 1u << T_GO;
+    if (p.const_es_enabled) m |= 1u << T_CONST_ES;
+    else if (p.rel_es_enabled) m |= 1u << T_REL_ES;
+    if (p.qa_enabled) m |= 1u << T_QA;
+    if (p.pid_enabled) m |= 1u << T_PID;
+    E.mask = m;
+    E.TTd.mask = m;
+    E.TTf.mask = m;
+    E.pid_split = p.pid_enabled != 0;
+}
+
+void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
+    if (!p || !t) throw Fail{XMD_ERR_ARG, "null params or topology"};
+    E.P = *p;
+    if (!(p->dt > 0)) throw Fail{XMD_ERR_ARG, "dt must be positive"};
+    if (p->precision != XMD_PREC_FP64 && p->precision != XMD_PREC_MIXED) throw Fail{XMD_ERR_ARG, "bad precision"};
+    if (t->n_replicas <= 0 || t->n_beads <= 0) throw Fail{XMD_ERR_ARG, "empty topology"};
+    if (t->replica_offset[0] != 0 || t->replica_offset[t->n_replicas] != t->n_beads)
+        throw Fail{XMD_ERR_ARG, "replica_offset must span [0, n_beads]"};
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Fail{XMD_ERR_NO_DEVICE, "no HIP device"};
+    HIPCHK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+    E.nrep = t->n_replicas;
+    E.nbeads = t->n_beads;
+    E.nchains = t->n_chains;
+    E.rep_off.assign(t->replica_offset, t->replica_offset + E.nrep + 1);
+    E.rep_n.resize(E.nrep);
+    E.rep_slot_off.resize(E.nrep + 1);
+    int so = 0;
+    for (int k = 0; k < E.nrep; ++k) {
+        E.rep_n[k] = E.rep_off[k + 1] - E.rep_off[k];
+        if (E.rep_n[k] <= 0) throw Fail{XMD_ERR_ARG, "empty replica"};
+        E.rep_slot_off[k] = so;
+        so += round_up(E.rep_n[k], WAVE);
+    }
+    E.rep_slot_off[E.nrep] = so;
+    E.nslot = round_up(std::max(so, BLK), BLK);
+    const int n = E.nslot;
+    E.slot_of_bead.resize(E.nbeads);
+    E.bead_of_slot.assign(n, -1);
+    E.slot_rep_h.assign(n, 0);
+    for (int k = 0; k < E.nrep; ++k) {
+        int end = (k + 1 < E.nrep) ? E.rep_slot_off[k + 1] : n;
+        for (int s = E.rep_slot_off[k]; s < end; ++s) E.slot_rep_h[s] = k;
+        for (int i = 0; i < E.rep_n[k]; ++i) {
+            E.slot_of_bead[E.rep_off[k] + i] = E.rep_slot_off[k] + i;
+            E.bead_of_slot[E.rep_slot_off[k] + i] = E.rep_off[k] + i;
+        }
+    }
+    // chains
+    std::vector<int> chain_of(E.nbeads, -1);
+    for (int c = 0; c < t->n_chains; ++c)
+        for (int i = t->chain_offset[c]; i < t->chain_offset[c + 1]; ++i) {
+            if (i < 0 || i >= E.nbeads) throw Fail{XMD_ERR_ARG, "chain_offset out of range"};
+            chain_of[i] = c;
+        }
+    std::vector<int> chain_s(n, -1);
+    std::vector<uint8_t> bfl(n, 0);
+    std::vector<int8_t> at(n, 0);
+    for (int g = 0; g < E.nbeads; ++g) {
+        if (chain_of[g] < 0) throw Fail{XMD_ERR_ARG, "bead outside every chain"};
+        if (t->atype[g] < 0 || t->atype[g] >= XMD_NUM_AA) throw Fail{XMD_ERR_ARG, "atype out of range"};
+        int s = E.slot_of_bead[g];
+        chain_s[s] = chain_of[g];
+        at[s] = t->atype[g];
+        uint8_t f = B_ACTIVE;
+        if (g > 0 && chain_of[g - 1] == chain_of[g]) f |= B_PREV;
+        if (g + 1 < E.nbeads && chain_of[g + 1] == chain_of[g]) f |= B_NEXT;
+        int k = E.slot_rep_h[s];
+        if (((f & B_PREV) && g - 1 < E.rep_off[k]) || ((f & B_NEXT) && g + 1 >= E.rep_off[k + 1]))
+            throw Fail{XMD_ERR_ARG, "chain spans replicas"};
+        bfl[s] = f;
+    }
+    derive(E, t);
+    const xmd_params &P = E.P;
+    // per-replica arrays
+    std::vector<Box> boxes(E.nrep);
+    std::vector<double> nsd(E.nrep), temp(E.nrep), t0(E.nrep, 0.0);
+    std::vector<uint64_t> rng(E.nrep);
+    std::vector<int> cell_off(E.nrep + 1, 0), cell_dim(3 * E.nrep, 0);
+    int ncells = 0;
+    for (int k = 0; k < E.nrep; ++k) {
+        Box b{t->box[3 * k], t->box[3 * k + 1], t->box[3 * k + 2], 0, 0, 0};
+        if (b.cx != 0.0) b.ix = 1.0 / b.cx;
+        if (b.cy != 0.0) b.iy = 1.0 / b.cy;
+        if (b.cz != 0.0) b.iz = 1.0 / b.cz;
+        boxes[k] = b;
+        temp[k] = t->temperature[k];
+        // lang_pc.cpp:7-13 with mass 1
+        double noise_factor = std::sqrt(2.0 * 1.0 * temp[k]);
+        double gfs = std::sqrt(P.gamma_factor);
+        nsd[k] = noise_factor * gfs * 1.0;
+        rng[k] = t->rng_state[k];
+        cell_off[k] = ncells;
+        double side = E.G.req_r * (1.0 + 1e-9);
+        int nx = b.cx > 0 ? (int)std::floor(b.cx / side) : 0;
+        int ny = b.cy > 0 ? (int)std::floor(b.cy / side) : 0;
+        int nz = b.cz > 0 ? (int)std::floor(b.cz / side) : 0;
+        if (nx >= 3 && ny >= 3 && nz >= 3 && (long long)nx * ny * nz <= 64LL * E.rep_n[k] + 4096) {
+            cell_dim[3 * k] = nx;
+            cell_dim[3 * k + 1] = ny;
+            cell_dim[3 * k + 2] = nz;
+            ncells += nx * ny * nz;
+        }
+    }
+    cell_off[E.nrep] = ncells;
+    // native contacts: rows of both ends, sorted
+    std::vector<std::vector<std::pair<int, double>>> nrow(n);
+    for (int c = 0; c < t->n_native; ++c) {
+        int a = t->nat_i1[c], b = t->nat_i2[c];
+        if (a < 0 || b >= E.nbeads || !(a < b)) throw Fail{XMD_ERR_ARG, "native contact needs 0 <= i1 < i2 < n"};
+        int sa = E.slot_of_bead[a], sb = E.slot_of_bead[b];
+        if (E.slot_rep_h[sa] != E.slot_rep_h[sb]) throw Fail{XMD_ERR_ARG, "native contact crosses replicas"};
+        nrow[sa].push_back({sb, t->nat_dist[c]});
+        nrow[sb].push_back({sa, t->nat_dist[c]});
+    }
+    std::vector<int> nat_start(n + 1, 0), nat_j;
+    std::vector<double> nat_d;
+    for (int s = 0; s < n; ++s) {
+        std::sort(nrow[s].begin(), nrow[s].end());
+        nat_start[s] = (int)nat_j.size();
+        for (auto &e : nrow[s]) {
+            nat_j.push_back(e.first);
+            nat_d.push_back(e.second);
+        }
+    }
+    nat_start[n] = (int)nat_j.size();
+    // bonded natives in slot space
+    std::vector<double> teth(n, P.tether_def_length), th(n, NAN), ph(n, NAN), ch(n, NAN), cf(n, NAN);
+    for (int g = 0; g < E.nbeads; ++g) {
+        int s = E.slot_of_bead[g];
+        if (t->tether_len && !std::isnan(t->tether_len[g])) teth[s] = t->tether_len[g];
+        if (t->nat_theta) th[s] = t->nat_theta[g];
+        if (t->nat_phi) ph[s] = t->nat_phi[g];
+        if (t->nat_chir) ch[s] = t->nat_chir[g];
+        if (t->nat_chir_factor) cf[s] = t->nat_chir_factor[g];
+    }
+    // positions and state
+    std::vector<double> pos(3 * n, 0.0);
+    for (int g = 0; g < E.nbeads; ++g) {
+        int s = E.slot_of_bead[g];
+        for (int d = 0; d < 3; ++d) pos[d * n + s] = t->r0[d * E.nbeads + g];
+    }
+    // QA sync (D7)
+    std::vector<uint32_t> sync(n, 0);
+    for (int s = 0; s < n; ++s)
+        if (bfl[s] & B_ACTIVE) {
+            const int8_t *L = P.aa_limits[at[s]];
+            sync[s] = xmd_dev::sync_pack(L[0], L[1], L[2], L[3]);
+        }
+    // tables
+    std::vector<double> ss_depth(P.ss_depth, P.ss_depth + XMD_NUM_SS), ss_rmax(P.ss_rmax, P.ss_rmax + XMD_NUM_SS),
+        ss_rmin(XMD_NUM_SS);
+    for (int k = 0; k < XMD_NUM_SS; ++k) ss_rmin[k] = P.ss_sinking ? P.lj_bb_rmin : ss_rmax[k];
+    std::vector<float> ssdf(XMD_NUM_SS), ssnf(XMD_NUM_SS), ssxf(XMD_NUM_SS);
+    for (int k = 0; k < XMD_NUM_SS; ++k) {
+        ssdf[k] = (float)ss_depth[k];
+        ssnf[k] = (float)ss_rmin[k];
+        ssxf[k] = (float)ss_rmax[k];
+    }
+    std::vector<double> rmd(XMD_QA_NUM_TYPES, 0.0);
+    rmd[1] = P.lj_bb_rmin;
+    rmd[2] = P.lj_bs_rmin;
+    rmd[3] = P.lj_bs_rmin;
+    for (int a1 = 0; a1 < XMD_NUM_AA; ++a1)
+        for (int a2 = 0; a2 < XMD_NUM_AA; ++a2) rmd[4 + a1 + a2 * XMD_NUM_AA] = ss_rmax[a1 * XMD_NUM_AA + a2];
+    std::vector<double> charge(P.aa_charge, P.aa_charge + XMD_NUM_AA);
+    std::vector<int8_t> ptype(P.aa_polarization, P.aa_polarization + XMD_NUM_AA);
+    Mat J = step_matrix();
+    std::vector<uint64_t> jrow(64 * 24), jlane(64 * 64);
+    Mat J256 = mpow(J, 256);
+    for (int j = 0; j < 24; ++j) {
+        rows_of(J256, jrow.data() + 64 * j);
+        J256 = mul(J256, J256);
+    }
+    Mat J4 = mpow(J, 4), Jl = identity();
+    for (int l = 0; l < 64; ++l) {
+        for (int b = 0; b < 64; ++b) jlane[64 * b + l] = Jl.col[b];
+        Jl = mul(J4, Jl);
+    }
+
+    Dev &D = E.D;
+    std::memset(&D, 0, sizeof(D));
+    D.nslot = n;
+    D.nrep = E.nrep;
+    D.rep_slot_off = E.upload(E.rep_slot_off);
+    D.rep_n = E.upload(E.rep_n);
+    D.box = E.upload(boxes);
+    D.noise_sd = E.upload(nsd);
+    D.rng[0] = E.upload(rng);
+    D.rng[1] = E.upload(rng);
+    D.tt[0] = E.upload(t0);
+    D.tt[1] = E.upload(t0);
+    D.maxd[0] = E.alloc<unsigned long long>(E.nrep);
+    D.maxd[1] = E.alloc<unsigned long long>(E.nrep);
+    D.rb_now = E.alloc<int>(E.nrep);
+    D.force_rb = E.alloc<int>(E.nrep);
+    D.any_rb = E.alloc<int>(2);
+    D.err = E.alloc<int>(1);
+    D.n_rebuild = E.alloc<long long>(E.nrep);
+    D.cell_off = E.upload(cell_off);
+    D.cell_dim = E.upload(cell_dim);
+    D.cell_cnt = E.alloc<int>(ncells);
+    D.cell_start = E.alloc<int>(ncells);
+    D.cell_end = E.alloc<int>(ncells);
+    D.cell_cursor = E.alloc<int>(ncells);
+    D.cell_beads = E.alloc<int>(n);
+    D.bead_cell = E.alloc<int>(n);
+    D.slot_rep = E.upload(E.slot_rep_h);
+    D.chain = E.upload(chain_s);
+    D.bflags = E.upload(bfl);
+    D.atype = E.upload(at);
+    D.pos[0] = E.upload(pos);
+    D.pos[1] = E.upload(pos);
+    D.y = E.alloc<double>(15 * (size_t)n);
+    D.orig_r = E.alloc<double>(3 * (size_t)n);
+    D.Fbuf = E.alloc<double>(3 * (size_t)n);
+    D.pid_side = E.alloc<double>(6 * (size_t)n);
+    D.Vbead = E.alloc<double>(n);
+    D.Vterm = E.alloc<double>((size_t)NUM_TERMS * n);
+    D.tether_nat = E.upload(teth);
+    D.theta_nat = E.upload(th);
+    D.phi_nat = E.upload(ph);
+    D.chir_nat = E.upload(ch);
+    D.chir_fac = E.upload(cf);
+    D.nat_start = E.upload(nat_start);
+    D.nat_j = E.upload(nat_j);
+    D.nat_d = E.upload(nat_d);
+    D.nat_act = E.alloc<uint8_t>(nat_j.size());
+    D.sync = E.upload(sync);
+    D.qslot = E.alloc<int>((size_t)MAXQ * n);
+    D.qn = E.alloc<int>(n);
+    D.ss_depth = E.upload(ss_depth);
+    D.ss_rmin = E.upload(ss_rmin);
+    D.ss_rmax = E.upload(ss_rmax);
+    D.ss_depth_f = E.upload(ssdf);
+    D.ss_rmin_f = E.upload(ssnf);
+    D.ss_rmax_f = E.upload(ssxf);
+    D.req_min_dist = E.upload(rmd);
+    D.charge = E.upload(charge);
+    D.ptype = E.upload(ptype);
+    D.jrow = E.upload(jrow);
+    D.jlane = E.upload(jlane);
+    if (P.rng_exact) D.bm_fix = E.upload(fix_table());
+    int nsl = n / WAVE;
+    D.slice_start = E.alloc<int>(nsl);
+    D.slice_w = E.alloc<int>(nsl);
+    D.nl_len = E.alloc<int>(n);
+    D.nl_half = E.alloc<int>(n);
+    D.fp_slice_cnt = E.alloc<int>(nsl);
+    D.fp_slice_start = E.alloc<int>(nsl);
+    // QA contacts: at most MAXQ per bead -> 8n live; twice that for vacancies
+    E.ct_base_h.resize(E.nrep + 1);
+    E.ct_cap_h.resize(E.nrep + 1);
+    int ct = 0;
+    for (int k = 0; k < E.nrep; ++k) {
+        E.ct_base_h[k] = ct;
+        E.ct_cap_h[k] = P.qa_enabled ? round_up(16 * E.rep_n[k] + 1024, BLK) : BLK;
+        ct += E.ct_cap_h[k];
+    }
+    E.ct_base_h[E.nrep] = ct;
+    E.ct_cap_h[E.nrep] = ct;  // total, read by kernels as the SoA stride
+    E.ct_total = ct;
+    D.ct_base = E.upload(E.ct_base_h);
+    D.ct_cap = E.upload(E.ct_cap_h);
+    std::vector<int> ctrep(ct / BLK, -1);
+    for (int k = 0; k < E.nrep; ++k)
+        for (int c = E.ct_base_h[k] / BLK; c < (E.ct_base_h[k] + E.ct_cap_h[k]) / BLK; ++c) ctrep[c] = k;
+    D.ct_chunk_rep = E.upload(ctrep);
+    E.nct_chunks = ct / BLK;
+    D.ct_count = E.alloc<int>(E.nrep);
+    D.ct_i1 = E.alloc<int>(ct);
+    D.ct_i2 = E.alloc<int>(ct);
+    D.ct_type = E.alloc<int>(ct);
+    D.ct_status = E.alloc<int>(ct);
+    D.ct_ref = E.alloc<double>(ct);
+    D.ct_F = E.alloc<double>(3 * (size_t)ct);
+    D.ct_V = E.alloc<double>(ct);
+    D.ct_d1 = E.alloc<uint32_t>(ct);
+    D.ct_d2 = E.alloc<uint32_t>(ct);
+    D.ct_vac = E.alloc<uint8_t>(ct);
+    D.rm_cnt = E.alloc<int>(ct / BLK);
+    D.rm_idx = E.alloc<int>(ct);
+
+    // first rebuild, sized in two passes: count rows, then allocate rows
+    E.pending_rb.assign(E.nrep, 1);
+    E.put(D.force_rb, std::vector<int>(E.nrep, 1));
+    // placeholders so kernels see valid pointers for the count pass
+    std::vector<int> zero_base(E.nrep + 1, 0);
+    D.nl_base = E.upload(zero_base);
+    D.nl_cap = E.upload(zero_base);
+    D.fp_base = E.upload(zero_base);
+    D.fp_cap = E.upload(zero_base);
+    D.fp_count = E.alloc<int>(E.nrep);
+    k_finalize<<<E.nrep, WAVE, 0, E.stream>>>(D, E.G, 0, 0);
+    int nb = E.nblk();
+    k_cell_count<<<nb, BLK, 0, E.stream>>>(D, D.pos[0], 0);
+    k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 0);
+    k_cell_fill<<<nb, BLK, 0, E.stream>>>(D, 0);
+    k_nl_count<<<nb, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
+    HIPCHK(hipGetLastError());
+    auto w = E.get(D.slice_w, nsl);
+    E.nl_base_h.resize(E.nrep + 1);
+    E.nl_cap_h.resize(E.nrep + 1);
+    E.fp_base_h.resize(E.nrep + 1);
+    E.fp_cap_h.resize(E.nrep + 1);
+    long long nl = 0, fp = 0;
+    for (int k = 0; k < E.nrep; ++k) {
+        long long need = 0;
+        for (int sl = E.rep_slot_off[k] / WAVE; sl < (E.rep_slot_off[k] + E.rep_n[k] + WAVE - 1) / WAVE; ++sl)
+            need += 64LL * w[sl];
+        long long cap = round_up((int)std::min<long long>(2 * need + 64LL * 64, 1LL << 30), BLK);
+        E.nl_base_h[k] = (int)nl;
+        E.nl_cap_h[k] = (int)cap;
+        nl += cap;
+        long long fcap = P.qa_enabled ? round_up((int)(cap / 2 + E.ct_cap_h[k]), BLK) : BLK;
+        E.fp_base_h[k] = (int)fp;
+        E.fp_cap_h[k] = (int)fcap;
+        fp += fcap;
+    }
+    if (nl >= (1LL << 31) || fp >= (1LL << 31)) throw Fail{XMD_ERR_CAPACITY, "system too large for int32 list offsets"};
+    E.nl_base_h[E.nrep] = (int)nl;
+    E.nl_cap_h[E.nrep] = (int)nl;
+    E.fp_base_h[E.nrep] = (int)fp;
+    E.fp_cap_h[E.nrep] = (int)fp;
+    E.nl_total = (int)nl;
+    E.fp_total = (int)fp;
+    E.put(const_cast<int *>(D.nl_base), E.nl_base_h);
+    E.put(const_cast<int *>(D.nl_cap), E.nl_cap_h);
+    E.put(const_cast<int *>(D.fp_base), E.fp_base_h);
+    E.put(const_cast<int *>(D.fp_cap), E.fp_cap_h);
+    D.nl_j = E.alloc<int>(nl);
+    D.nl_flag = E.alloc<uint8_t>(nl);
+    D.fp_i1 = E.alloc<int>(fp);
+    D.fp_i2 = E.alloc<int>(fp);
+    D.fp_vac = E.alloc<uint8_t>(fp);
+    D.cand_cnt = E.alloc<int>(fp / BLK);
+    D.cand_fp = E.alloc<int>(fp);
+    D.cand_type = E.alloc<int>(fp);
+    D.cand_d1 = E.alloc<uint32_t>(fp);
+    D.cand_d2 = E.alloc<uint32_t>(fp);
+    std::vector<int> fprep(fp / BLK, -1);
+    for (int k = 0; k < E.nrep; ++k)
+        for (int c = E.fp_base_h[k] / BLK; c < (E.fp_base_h[k] + E.fp_cap_h[k]) / BLK; ++c) fprep[c] = k;
+    D.fp_chunk_rep = E.upload(fprep);
+    E.nfp_chunks = (int)(fp / BLK);
+    E.qa_grid = std::max(1, std::min(E.nfp_chunks + E.nct_chunks, 8192));
+    // finish the first rebuild
+    k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 1);
+    k_nl_fill<<<nb, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
+    k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 2);
+    k_fp_write<<<nb, BLK, 0, E.stream>>>(D, 0);
+    HIPCHK(hipGetLastError());
+    std::fill(E.pending_rb.begin(), E.pending_rb.end(), 0);
+    E.check_err();
+}
+
+xmd_status run_guarded(std::string &err, const std::function<void()> &fn) {
+    try {
+        fn();
+        return XMD_OK;
+    } catch (const Fail &f) {
+        err = f.msg;
+        return f.st;
+    } catch (const std::exception &ex) {
+        err = ex.what();
+        return XMD_ERR_STATE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char *xmd_version(void) { return "xmd-hip 0.1.0 gfx950"; }
+
+const char *xmd_last_error(const xmd_engine *e) { return e ? e->err.c_str() : g_create_err.c_str(); }
+
+xmd_status xmd_engine_create(const xmd_params *p, const xmd_topology *t, xmd_engine **out) {
+    if (!out) {
+        g_create_err = "null output pointer";
+        return XMD_ERR_ARG;
+    }
+    *out = nullptr;
+    xmd_engine *e = new xmd_engine();
+    xmd_status st = run_guarded(g_create_err, [&]() { build(*e, p, t); });
+    if (st != XMD_OK) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return XMD_OK;
+}
+
+void xmd_engine_destroy(xmd_engine *e) { delete e; }
+
+void *xmd_stream(xmd_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+xmd_status xmd_set_state(xmd_engine *e, int32_t rep, const double *y, uint64_t rng, double t) {
+    if (!e) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep || !y) throw Fail{XMD_ERR_ARG, "bad replica or null state"};
+        int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep], n = e->nslot;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        for (int k = 0; k < 6; ++k)
+            for (int d = 0; d < 3; ++d) {
+                const double *src = y + (3 * k + d) * nb;
+                double *dst = k == 0 ? e->D.pos[e->cur] + (size_t)d * n + s0 : e->D.y + (size_t)(3 * (k - 1) + d) * n + s0;
+                HIPCHK(hipMemcpy(dst, src, nb * sizeof(double), hipMemcpyHostToDevice));
+            }
+        HIPCHK(hipMemcpy(e->D.rng[e->cur] + rep, &rng, sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->D.tt[e->cur] + rep, &t, sizeof(double), hipMemcpyHostToDevice));
+        e->pending_rb[rep] = 1;
+    });
+}
+
+xmd_status xmd_get_state(xmd_engine *e, int32_t rep, double *y, double *v, uint64_t *rng, double *t) {
+    if (!e) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
+        int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep], n = e->nslot;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        std::vector<double> y1(3 * (size_t)nb);
+        for (int k = 0; k < 6; ++k)
+            for (int d = 0; d < 3; ++d) {
+                const double *src = k == 0 ? e->D.pos[e->cur] + (size_t)d * n + s0
+                                           : e->D.y + (size_t)(3 * (k - 1) + d) * n + s0;
+                double *dst = y ? y + (3 * k + d) * nb : nullptr;
+                if (k == 1) {
+                    HIPCHK(hipMemcpy(y1.data() + d * nb, src, nb * sizeof(double), hipMemcpyDeviceToHost));
+                    if (dst) std::memcpy(dst, y1.data() + d * nb, nb * sizeof(double));
+                } else if (dst) {
+                    HIPCHK(hipMemcpy(dst, src, nb * sizeof(double), hipMemcpyDeviceToHost));
+                }
+            }
+        if (v)
+            for (size_t i = 0; i < y1.size(); ++i) v[i] = y1[i] * e->G.dt_inv;
+        if (rng) HIPCHK(hipMemcpy(rng, e->D.rng[e->cur] + rep, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (t) HIPCHK(hipMemcpy(t, e->D.tt[e->cur] + rep, sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+xmd_status xmd_set_positions(xmd_engine *e, int32_t rep, const double *r) {
+    if (!e) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep || !r) throw Fail{XMD_ERR_ARG, "bad replica or null positions"};
+        int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep], n = e->nslot;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        for (int d = 0; d < 3; ++d)
+            HIPCHK(hipMemcpy(e->D.pos[e->cur] + (size_t)d * n + s0, r + d * nb, nb * sizeof(double),
+                             hipMemcpyHostToDevice));
+        e->pending_rb[rep] = 1;
+    });
+}
+
+xmd_status xmd_rebuild_lists(xmd_engine *e) {
+    if (!e) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        std::fill(e->pending_rb.begin(), e->pending_rb.end(), 1);
+        e->ensure_lists();
+    });
+}
+
+xmd_status xmd_get_list(xmd_engine *e, int32_t rep, int32_t id, int32_t *i1, int32_t *i2, int64_t cap, int64_t *nout) {
+    if (!e || !nout) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
+        e->ensure_lists();
+        int s0 = e->rep_slot_off[rep], nb = e->rep_n[rep], n = e->nslot;
+        std::vector<std::pair<int, int>> out;
+        auto local = [&](int s) { return s - s0; };
+        if (id == XMD_LIST_NL_ALL || id == XMD_LIST_NL_NATIVE || id == XMD_LIST_NL_NON_NATIVE) {
+            int *cnt = e->alloc<int>(n);
+            k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt, nullptr, nullptr, nullptr, nullptr, 0);
+            auto c = e->get(cnt, n);
+            std::vector<int> oall(n + 1, 0);
+            for (int s = 0; s < n; ++s) oall[s + 1] = oall[s] + c[s];
+            int *off = e->upload(oall);
+            int *oi1 = e->alloc<int>(oall[n] + 1), *oi2 = e->alloc<int>(oall[n] + 1);
+            uint8_t *onat = e->alloc<uint8_t>(oall[n] + 1);
+            k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt, off, oi1, oi2, onat, 1);
+            HIPCHK(hipGetLastError());
+            auto a = e->get(oi1, oall[n]), b = e->get(oi2, oall[n]);
+            auto nat = e->get(onat, oall[n]);
+            for (int s = s0; s < s0 + nb; ++s)
+                for (int k = oall[s]; k < oall[s + 1]; ++k) {
+                    bool keep = id == XMD_LIST_NL_ALL || (id == XMD_LIST_NL_NATIVE ? nat[k] : !nat[k]);
+                    if (keep) out.push_back({local(a[k]), local(b[k])});
+                }
+        } else if (id == XMD_LIST_GO) {
+            auto st = e->get(e->D.nat_start, n + 1);
+            auto j = e->get(e->D.nat_j, st[n]);
+            auto act = e->get(e->D.nat_act, st[n]);
+            for (int s = s0; s < s0 + nb; ++s)
+                for (int k = st[s]; k < st[s + 1]; ++k)
+                    if (j[k] > s && act[k]) out.push_back({local(s), local(j[k])});
+        } else if (id == XMD_LIST_PAULI || id == XMD_LIST_ES || id == XMD_LIST_PID) {
+            uint8_t want = id == XMD_LIST_PAULI ? F_PAULI : (id == XMD_LIST_ES ? F_ES : F_PID);
+            auto len = e->get(e->D.nl_len, n);
+            auto ss = e->get(e->D.slice_start, n / WAVE);
+            auto j = e->get(e->D.nl_j, e->nl_total);
+            auto fl = e->get(e->D.nl_flag, e->nl_total);
+            for (int s = s0; s < s0 + nb; ++s)
+                for (int k = 0; k < len[s]; ++k) {
+                    int idx = ss[s / WAVE] + 64 * k + s % WAVE;
+                    if (j[idx] > s && (fl[idx] & want)) out.push_back({local(s), local(j[idx])});
+                }
+        } else if (id == XMD_LIST_QA_FREE) {
+            auto cntv = e->get(e->D.fp_count, e->nrep);
+            int b = e->fp_base_h[rep];
+            auto a = e->get(e->D.fp_i1 + b, cntv[rep]);
+            auto c = e->get(e->D.fp_i2 + b, cntv[rep]);
+            auto v = e->get(e->D.fp_vac + b, cntv[rep]);
+            for (int k = 0; k < cntv[rep]; ++k)
+                if (!v[k]) out.push_back({local(a[k]), local(c[k])});
+        } else {
+            throw Fail{XMD_ERR_ARG, "bad list id"};
+        }
+        *nout = (int64_t)out.size();
+        if (i1 && i2)
+            for (int64_t k = 0; k < std::min<int64_t>(cap, *nout); ++k) {
+                i1[k] = out[k].first;
+                i2[k] = out[k].second;
+            }
+    });
+}
+
+xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double *V_terms) {
+    if (!e || !F) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        e->ensure_lists();
+        uint32_t m = term_mask & e->mask;
+        int p = e->cur;
+        if (e->G.qa_on && (m & (1u << T_QA)))
+            k_qa_pairs<<<e->qa_grid, BLK, 0, e->stream>>>(e->D, e->G, p, e->nfp_chunks, e->nct_chunks, 0);
+        int nb = e->nblk();
+        if (e->P.precision == XMD_PREC_MIXED) {
+            TermTab<float> TT = e->TTf;
+            TT.mask = m;
+            k_force<float, false, true><<<nb, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
+        } else {
+            TermTab<double> TT = e->TTd;
+            TT.mask = m;
+            k_force<double, false, true><<<nb, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
+        }
+        k_gather_force<<<nb, BLK, 0, e->stream>>>(e->D);
+        HIPCHK(hipGetLastError());
+        int n = e->nslot;
+        auto Fh = e->get(e->D.Fbuf, 3 * (size_t)n);
+        for (int g = 0; g < e->nbeads; ++g) {
+            int s = e->slot_of_bead[g];
+            for (int d = 0; d < 3; ++d) F[d * e->nbeads + g] = Fh[d * n + s];
+        }
+        if (V_terms) {
+            auto Vt = e->get(e->D.Vterm, (size_t)NUM_TERMS * n);
+            for (int k = 0; k < e->nrep; ++k)
+                for (int t = 0; t < NUM_TERMS; ++t) {
+                    double acc = 0.0;
+                    if (m & (1u << t))
+                        for (int s = e->rep_slot_off[k]; s < e->rep_slot_off[k] + e->rep_n[k]; ++s)
+                            acc += Vt[(size_t)t * n + s];
+                    V_terms[k * XMD_NUM_TERMS + t] = acc;
+                }
+        }
+    });
+}
+
+xmd_status xmd_qa_update(xmd_engine *e) {
+    if (!e) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        e->ensure_lists();
+        if (!e->G.qa_on) return;
+        k_qa_pairs<<<e->qa_grid, BLK, 0, e->stream>>>(e->D, e->G, e->cur, e->nfp_chunks, e->nct_chunks, 1);
+        k_qa_serial<<<e->nrep, BLK, 0, e->stream>>>(e->D, e->G, e->cur);
+        HIPCHK(hipGetLastError());
+        e->check_err();
+    });
+}
+
+xmd_status xmd_get_qa_state(xmd_engine *e, int32_t rep, xmd_qa_contact *c, int64_t cap, int64_t *nout, int8_t *sync) {
+    if (!e || !nout) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
+        int s0 = e->rep_slot_off[rep], nb = e->rep_n[rep];
+        auto cnt = e->get(e->D.ct_count, e->nrep)[rep];
+        int b = e->ct_base_h[rep];
+        auto i1 = e->get(e->D.ct_i1 + b, cnt), i2 = e->get(e->D.ct_i2 + b, cnt), ty = e->get(e->D.ct_type + b, cnt),
+             st = e->get(e->D.ct_status + b, cnt);
+        auto rf = e->get(e->D.ct_ref + b, cnt);
+        auto d1 = e->get(e->D.ct_d1 + b, cnt), d2 = e->get(e->D.ct_d2 + b, cnt);
+        auto vac = e->get(e->D.ct_vac + b, cnt);
+        int64_t k = 0;
+        for (int q = 0; q < cnt; ++q) {
+            if (vac[q]) continue;
+            if (c && k < cap) {
+                xmd_qa_contact &o = c[k];
+                o.i1 = i1[q] - s0;
+                o.i2 = i2[q] - s0;
+                o.type = (int16_t)ty[q];
+                o.status = (int8_t)st[q];
+                o._pad = 0;
+                std::memcpy(o.diff1, &d1[q], 4);
+                std::memcpy(o.diff2, &d2[q], 4);
+                o.ref_time = rf[q];
+            }
+            ++k;
+        }
+        *nout = k;
+        if (sync) {
+            auto sy = e->get(e->D.sync + s0, nb);
+            std::memcpy(sync, sy.data(), 4 * (size_t)nb);
+        }
+    });
+}
+
+xmd_status xmd_set_qa_state(xmd_engine *e, int32_t rep, const xmd_qa_contact *c, int64_t nc, const int8_t *sync) {
+    if (!e) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
+        int s0 = e->rep_slot_off[rep], nb = e->rep_n[rep], n = e->nslot;
+        if (nc > e->ct_cap_h[rep] / 2) throw Fail{XMD_ERR_CAPACITY, "too many contacts"};
+        std::vector<int> i1(nc), i2(nc), ty(nc), st(nc);
+        std::vector<double> rf(nc);
+        std::vector<uint32_t> d1(nc), d2(nc);
+        std::vector<uint8_t> vac(nc, 0);
+        std::vector<int> qn(nb, 0), qslot((size_t)MAXQ * nb, 0);
+        int b = e->ct_base_h[rep];
+        for (int64_t k = 0; k < nc; ++k) {
+            if (c[k].i1 < 0 || c[k].i2 >= nb || !(c[k].i1 < c[k].i2)) throw Fail{XMD_ERR_ARG, "bad contact"};
+            i1[k] = c[k].i1 + s0;
+            i2[k] = c[k].i2 + s0;
+            ty[k] = c[k].type;
+            st[k] = c[k].status;
+            rf[k] = c[k].ref_time;
+            std::memcpy(&d1[k], c[k].diff1, 4);
+            std::memcpy(&d2[k], c[k].diff2, 4);
+            for (int a : {c[k].i1, c[k].i2}) {
+                if (qn[a] >= MAXQ) throw Fail{XMD_ERR_CAPACITY, "more QA contacts on a bead than slots"};
+                qslot[(size_t)qn[a] * nb + a] = b + (int)k;
+                qn[a]++;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(e->stream));
+        auto up = [&](auto *dst, const auto &h) {
+            if (!h.empty()) HIPCHK(hipMemcpy(dst, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+        };
+        up(e->D.ct_i1 + b, i1);
+        up(e->D.ct_i2 + b, i2);
+        up(e->D.ct_type + b, ty);
+        up(e->D.ct_status + b, st);
+        up(e->D.ct_ref + b, rf);
+        up(e->D.ct_d1 + b, d1);
+        up(e->D.ct_d2 + b, d2);
+        up(e->D.ct_vac + b, vac);
+        int cnt = (int)nc;
+        HIPCHK(hipMemcpy(e->D.ct_count + rep, &cnt, sizeof(int), hipMemcpyHostToDevice));
+        up(e->D.qn + s0, qn);
+        for (int q = 0; q < MAXQ; ++q)
+            HIPCHK(hipMemcpy(e->D.qslot + (size_t)q * n + s0, qslot.data() + (size_t)q * nb, nb * sizeof(int),
+                             hipMemcpyHostToDevice));
+        if (sync) HIPCHK(hipMemcpy(e->D.sync + s0, sync, 4 * (size_t)nb, hipMemcpyHostToDevice));
+        e->pending_rb[rep] = 1;
+    });
+}
+
+xmd_status xmd_step(xmd_engine *e, int64_t nsteps) {
+    if (!e || nsteps < 0) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() { e->run_steps(nsteps); });
+}
+
+xmd_status xmd_get_energy(xmd_engine *e, double *V) {
+    if (!e || !V) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        auto Vb = e->get(e->D.Vbead, e->nslot);
+        for (int k = 0; k < e->nrep; ++k) {
+            double acc = 0.0;
+            for (int s = e->rep_slot_off[k]; s < e->rep_slot_off[k] + e->rep_n[k]; ++s) acc += Vb[s];
+            V[k] = acc;
+        }
+    });
+}
+
+xmd_status xmd_get_counters(xmd_engine *e, int64_t *out) {
+    if (!e || !out) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        auto rb = e->get(e->D.n_rebuild, e->nrep);
+        auto len = e->get(e->D.nl_len, e->nslot);
+        auto cnt = e->get(e->D.ct_count, e->nrep);
+        int64_t r = 0, nl = 0, qa = 0;
+        for (auto x : rb) r += x;
+        for (int s = 0; s < e->nslot; ++s)
+            if (e->bead_of_slot[s] >= 0) nl += len[s];
+        for (int k = 0; k < e->nrep; ++k) {
+            auto v = e->get(e->D.ct_vac + e->ct_base_h[k], cnt[k]);
+            for (auto x : v) qa += !x;
+        }
+        out[0] = e->steps;
+        out[1] = r;
+        out[2] = nl;
+        out[3] = e->nl_total;
+        out[4] = qa;
+    });
+}
+
+xmd_status xmd_rand_normals(xmd_engine *e, uint64_t state, int64_t nb, float *out, uint64_t *state_out) {
+    if (!e || nb < 0 || (nb > 0 && !out)) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        if (nb > (1LL << 28)) throw Fail{XMD_ERR_ARG, "too many beads"};
+        long long padded = (nb + BLK - 1) / BLK * BLK;
+        float *d = nullptr;
+        if (nb > 0) {
+            HIPCHK(hipMalloc(&d, 3 * sizeof(float) * (size_t)padded));
+            k_rand_normals<<<(int)(padded / BLK), BLK, 0, e->stream>>>(e->D, state, nb, d);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(out, d, 3 * sizeof(float) * (size_t)nb, hipMemcpyDeviceToHost, e->stream));
+            HIPCHK(hipStreamSynchronize(e->stream));
+            HIPCHK(hipFree(d));
+        }
+        if (state_out) {
+            Mat J = step_matrix();
+            *state_out = apply(mpow(J, 4ull * (uint64_t)nb), state);
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1024 @@
+// kernels.hpp -- HIP kernels of the step path (gfx950, wave64).
+//
+// Layout: beads live in "slots"; every replica occupies a 64-aligned slot
+// range, so no wavefront spans two replicas (the RNG jump-ahead and the
+// per-replica reductions are wave-uniform).  Neighbour rows are sliced
+// ELL: entry k of bead s sits at slice_start[s/64] + 64*k + s%64, so a
+// wavefront reads a row column with one coalesced load.
+//
+// Step sequence (parity p = step & 1; positions ping-pong pos[p] ->
+// pos[1-p] because neighbours read old positions while owners integrate):
+//   k_qa_pairs    sift free pairs + advance contacts      (qa/*.cpp)
+//   k_qa_serial   accept candidates, apply removals       (process_candidates.cpp)
+//   k_force       all force terms per bead (+ integrate)  (forces/*.cpp, lang_pc.cpp)
+//   k_integrate   when PID forces spill to neighbours
+//   k_finalize    RNG/time advance, verify decision       (nl/verify.cpp)
+//   k_cell_* / k_nl_* / k_fp_write  Verlet rebuild        (nl/legacy_update.cpp + update_X)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "xmd_dev.hpp"
+
+namespace xmd_k {
+using namespace xmd_dev;
+
+constexpr int WAVE = 64;
+constexpr int BLK = 256;
+constexpr int MAXQ = 16;
+constexpr int NUM_TERMS = 11;
+
+enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16 };
+enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
+enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16 };
+enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID };
+
+// Derived parameters in double (list and QA decisions, integrator).
+struct Derived {
+    double dt, dt_inv, gamma, half_dt2;
+    double pad, req_r;
+    double pauli_min_inv, es_min_inv, pid_min_inv, qa_min_inv;
+    int pauli_on, es_on, pid_on, qa_on, go_on;
+    int lang_on;
+    QaParams qa;
+};
+
+// Every device pointer of an engine (passed by value to kernels).
+struct Dev {
+    int nslot, nrep;
+    // replicas
+    const int *rep_slot_off, *rep_n;
+    const Box *box;
+    const double *noise_sd;
+    uint64_t *rng[2];
+    double *tt[2];
+    unsigned long long *maxd[2];
+    int *rb_now, *force_rb, *any_rb, *err;
+    long long *n_rebuild;
+    const int *cell_off, *cell_dim;  // [nrep+1], [3*nrep]
+    int *cell_cnt, *cell_start, *cell_end, *cell_cursor, *cell_beads, *bead_cell;
+    // slots
+    const int *slot_rep, *chain;
+    const uint8_t *bflags;
+    const int8_t *atype;
+    double *pos[2];
+    double *y;  // y1..y5, 15*nslot
+    double *orig_r, *Fbuf, *pid_side, *Vbead, *Vterm;
+    const double *tether_nat, *theta_nat, *phi_nat, *chir_nat, *chir_fac;
+    // native contacts (rows of slot s: nat_start[s]..nat_start[s+1])
+    const int *nat_start, *nat_j;
+    const double *nat_d;
+    uint8_t *nat_act;
+    // neighbour rows
+    const int *nl_base, *nl_cap;
+    int *slice_start, *slice_w, *nl_len, *nl_half, *nl_j;
+    uint8_t *nl_flag;
+    // QA free pairs, candidates
+    const int *fp_base, *fp_cap, *fp_chunk_rep;
+    int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt, *fp_slice_start;
+    uint8_t *fp_vac;
+    int *cand_cnt, *cand_fp, *cand_type;
+    uint32_t *cand_d1, *cand_d2;
+    // QA contacts, removals
+    const int *ct_base, *ct_cap, *ct_chunk_rep;
+    int *ct_count, *ct_i1, *ct_i2, *ct_type, *ct_status;
+    double *ct_ref, *ct_F, *ct_V;
+    uint32_t *ct_d1, *ct_d2;
+    uint8_t *ct_vac;
+    int *rm_cnt, *rm_idx;
+    uint32_t *sync;
+    int *qslot, *qn;
+    // tables
+    const double *ss_depth, *ss_rmin, *ss_rmax, *req_min_dist, *charge;
+    const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
+    const int8_t *ptype;
+    const uint8_t *bm_fix;
+    const uint64_t *jrow, *jlane;
+};
+
+__device__ inline v3<double> ld3(const double *a, int n, int s) { return {a[s], a[n + s], a[2 * n + s]}; }
+__device__ inline void st3(double *a, int n, int s, v3<double> v) {
+    a[s] = v.x;
+    a[n + s] = v.y;
+    a[2 * n + s] = v.z;
+}
+__device__ inline int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// ------------------------------------------------------------ RNG jumps ---
+// y = M x over GF(2), M given by 64 rows (lane r holds row r): every lane
+// ends with the same y.  Needs a full wavefront.
+__device__ inline uint64_t wave_matvec_rows(const uint64_t *rows, uint64_t x) {
+    uint64_t row = rows[lane_id()];
+    int bit = __popcll(row & x) & 1;
+    return (uint64_t)__ballot(bit);
+}
+// State of the wave's lane l at draw offset 4*(64*w + l) from S (uniform).
+__device__ inline uint64_t wave_lane_state(const Dev &D, uint64_t S, int w) {
+    for (int j = 0; w != 0; ++j, w >>= 1)
+        if (w & 1) S = wave_matvec_rows(D.jrow + 64 * j, S);
+    uint64_t y = 0;
+    int l = lane_id();
+    for (int b = 0; b < 64; ++b)
+        if ((S >> b) & 1ull) y ^= D.jlane[64 * b + l];
+    return y;
+}
+
+// -------------------------------------------------------------- QA pairs ---
+// Chunks [0, nfp_chunks): sift 256 free pairs each (qa/sift_candidates.cpp);
+// chunks after that: process 256 contacts each (qa/process_contacts.cpp).
+// Candidates and removals are compacted per chunk in index order, so the
+// serial pass sees them in set order without sorting.  Grid-stride over
+// chunks; chunks past a replica's live count are skipped.
+__global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int nfp_chunks, int nct_chunks, int mutate) {
+    __shared__ int wsum[BLK / WAVE];
+    const int tid = threadIdx.x;
+    const int n = D.nslot;
+    const double *P = D.pos[p];
+    const int ct_tot = D.ct_cap[D.nrep];
+    for (int blk = blockIdx.x; blk < nfp_chunks + nct_chunks; blk += gridDim.x) {
+        bool sift = blk < nfp_chunks;
+        if (sift && !mutate) continue;
+        int chunk = sift ? blk : blk - nfp_chunks;
+        int rep = sift ? D.fp_chunk_rep[chunk] : D.ct_chunk_rep[chunk];
+        if (rep < 0) continue;
+        int first = chunk * BLK - (sift ? D.fp_base[rep] : D.ct_base[rep]);
+        int count = sift ? D.fp_count[rep] : D.ct_count[rep];
+        if (first >= count) continue;
+        int idx = chunk * BLK + tid;
+        int local = first + tid;
+        Box box = D.box[rep];
+        bool flag = false;
+        int type = 0;
+        uint32_t d1 = 0, d2 = 0;
+        if (sift) {
+            if (local < count && !D.fp_vac[idx]) {
+                int i1 = D.fp_i1[idx], i2 = D.fp_i2[idx];
+                uint8_t f1 = D.bflags[i1], f2 = D.bflags[i2];
+                bool ok1 = (f1 & B_PREV) && (f1 & B_NEXT), ok2 = (f2 & B_PREV) && (f2 & B_NEXT);
+                v3<double> n1, h1, n2, h2;
+                v3<double> r1 = ld3(P, n, i1), r2 = ld3(P, n, i2);
+                qa_nh(ok1, ok1 ? ld3(P, n, i1 - 1) : r1, r1, ok1 ? ld3(P, n, i1 + 1) : r1, n1, h1);
+                qa_nh(ok2, ok2 ? ld3(P, n, i2 - 1) : r2, r2, ok2 ? ld3(P, n, i2 + 1) : r2, n2, h2);
+                v3<double> r12 = r_uv(box, r1, r2);
+                type = qa_sift(G.qa, D.req_min_dist, D.ptype, r12, n1, h1, n2, h2, D.atype[i1], D.atype[i2],
+                               D.sync[i1], D.sync[i2], d1, d2);
+                flag = type != 0;
+            }
+        } else {
+            if (local < count && !D.ct_vac[idx]) {
+                int i1 = D.ct_i1[idx], i2 = D.ct_i2[idx];
+                v3<double> r12 = r_uv(box, ld3(P, n, i1), ld3(P, n, i2));
+                double depth, rmin, rmax;
+                int ctype = D.ct_type[idx], status = D.ct_status[idx], next;
+                qa_lj(G.qa, D.ss_depth, D.ss_rmin, D.ss_rmax, ctype, depth, rmin, rmax);
+                double t = D.tt[p][rep];
+                FV<double> f = qa_contact(G.qa, depth, rmin, rmax, r12, t, D.ct_ref[idx], status, next);
+                D.ct_F[idx] = f.F.x;
+                D.ct_F[ct_tot + idx] = f.F.y;
+                D.ct_F[2 * ct_tot + idx] = f.F.z;
+                D.ct_V[idx] = f.V;
+                if (mutate) {
+                    if (next == 1 && status == 0) {
+                        D.ct_status[idx] = 1;
+                        D.ct_ref[idx] = t;
+                    }
+                    flag = next == 2;
+                }
+            }
+            if (!mutate) continue;
+        }
+        // ordered compaction of the chunk
+        unsigned long long m = __ballot(flag);
+        int w = tid / WAVE;
+        if (lane_id() == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int off = 0;
+        for (int k = 0; k < w; ++k) off += wsum[k];
+        int rank = off + __popcll(m & ((1ull << lane_id()) - 1ull));
+        int base = chunk * BLK;
+        if (flag) {
+            if (sift) {
+                D.cand_fp[base + rank] = idx;
+                D.cand_type[base + rank] = type;
+                D.cand_d1[base + rank] = d1;
+                D.cand_d2[base + rank] = d2;
+            } else {
+                D.rm_idx[base + rank] = idx;
+            }
+        }
+        if (tid == 0) {
+            int tot = 0;
+            for (int k = 0; k < BLK / WAVE; ++k) tot += wsum[k];
+            if (sift) D.cand_cnt[chunk] = tot;
+            else D.rm_cnt[chunk] = tot;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ inline void slot_add(const Dev &D, int s, int ci) {
+    int q = D.qn[s];
+    if (q >= MAXQ) {
+        atomicOr(D.err, E_SLOT);
+        return;
+    }
+    D.qslot[q * D.nslot + s] = ci;
+    D.qn[s] = q + 1;
+}
+__device__ inline void slot_remove(const Dev &D, int s, int ci) {
+    int q = D.qn[s], k = 0;
+    while (k < q && D.qslot[k * D.nslot + s] != ci) ++k;
+    if (k == q) return;
+    for (; k + 1 < q; ++k) D.qslot[k * D.nslot + s] = D.qslot[(k + 1) * D.nslot + s];
+    D.qn[s] = q - 1;
+}
+
+// One workgroup per replica: process candidates in free-pair order, then
+// the removals of process_contacts in contact order, then compact the
+// contact set when its extent passes half the capacity.
+__global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
+    __shared__ int s_scan[BLK];
+    int rep = blockIdx.x, tid = threadIdx.x;
+    int fpb = D.fp_base[rep], ctb = D.ct_base[rep];
+    int nfpc = (D.fp_count[rep] + BLK - 1) / BLK;
+    double t = D.tt[p][rep];
+    // candidates
+    if (tid == 0) {
+        int cnt = D.ct_count[rep];
+        int cnt0 = cnt;
+        int cap = D.ct_cap[rep];
+        for (int c = 0; c < nfpc; ++c) {
+            int chunk = fpb / BLK + c;
+            int nc = D.cand_cnt[chunk];
+            for (int k = 0; k < nc; ++k) {
+                int e = chunk * BLK + k;
+                int fp = D.cand_fp[e];
+                int i1 = D.fp_i1[fp], i2 = D.fp_i2[fp];
+                uint32_t d1 = D.cand_d1[e], d2 = D.cand_d2[e];
+                uint32_t ns1 = sync_sub(D.sync[i1], d1), ns2 = sync_sub(D.sync[i2], d2);
+                if (sync_valid(ns1) && sync_valid(ns2)) {
+                    if (cnt >= cap) {
+                        atomicOr(D.err, E_CT_CAP);
+                        break;
+                    }
+                    int ci = ctb + cnt++;
+                    D.ct_i1[ci] = i1;
+                    D.ct_i2[ci] = i2;
+                    D.ct_type[ci] = D.cand_type[e];
+                    D.ct_status[ci] = 0;
+                    D.ct_ref[ci] = t;
+                    D.ct_d1[ci] = d1;
+                    D.ct_d2[ci] = d2;
+                    D.ct_vac[ci] = 0;
+                    int tot = D.ct_cap[D.nrep];
+                    D.ct_F[ci] = 0;
+                    D.ct_F[tot + ci] = 0;
+                    D.ct_F[2 * tot + ci] = 0;
+                    D.ct_V[ci] = 0;
+                    D.sync[i1] = ns1;
+                    D.sync[i2] = sync_sub(D.sync[i2], d2);
+                    slot_add(D, i1, ci);
+                    slot_add(D, i2, ci);
+                    D.fp_vac[fp] = 1;
+                }
+            }
+            D.cand_cnt[chunk] = 0;
+        }
+        // removals, in contact order (contacts formed above have no removal)
+        int nctc = (cnt0 + BLK - 1) / BLK;
+        int fcnt = D.fp_count[rep];
+        int fcap = D.fp_cap[rep];
+        for (int c = 0; c < nctc; ++c) {
+            int chunk = ctb / BLK + c;
+            int nr = D.rm_cnt[chunk];
+            for (int k = 0; k < nr; ++k) {
+                int ci = D.rm_idx[chunk * BLK + k];
+                int i1 = D.ct_i1[ci], i2 = D.ct_i2[ci];
+                D.ct_vac[ci] = 1;
+                D.sync[i1] = sync_add(D.sync[i1], D.ct_d1[ci]);
+                D.sync[i2] = sync_add(D.sync[i2], D.ct_d2[ci]);
+                if (fcnt >= fcap) {
+                    atomicOr(D.err, E_FP_CAP);
+                } else {
+                    int f = fpb + fcnt++;
+                    D.fp_i1[f] = i1;
+                    D.fp_i2[f] = i2;
+                    D.fp_vac[f] = 0;
+                }
+                slot_remove(D, i1, ci);
+                slot_remove(D, i2, ci);
+            }
+            D.rm_cnt[chunk] = 0;
+        }
+        D.fp_count[rep] = fcnt;
+        D.ct_count[rep] = cnt;
+        s_scan[0] = cnt > D.ct_cap[rep] / 2;
+    }
+    __syncthreads();
+    if (!s_scan[0]) return;
+    __syncthreads();
+    // compaction: in-place, chunk by chunk in index order (new <= old)
+    int cnt = D.ct_count[rep];
+    int kept = 0;
+    int tot = D.ct_cap[D.nrep];
+    for (int c0 = 0; c0 < cnt; c0 += BLK) {
+        int o = c0 + tid;
+        bool keep = o < cnt && !D.ct_vac[ctb + o];
+        int i1 = 0, i2 = 0, ty = 0, st = 0;
+        double rf = 0, fx = 0, fy = 0, fz = 0, v = 0;
+        uint32_t a = 0, b = 0;
+        if (keep) {
+            int ci = ctb + o;
+            i1 = D.ct_i1[ci]; i2 = D.ct_i2[ci]; ty = D.ct_type[ci]; st = D.ct_status[ci];
+            rf = D.ct_ref[ci]; a = D.ct_d1[ci]; b = D.ct_d2[ci];
+            fx = D.ct_F[ci]; fy = D.ct_F[tot + ci]; fz = D.ct_F[2 * tot + ci]; v = D.ct_V[ci];
+        }
+        s_scan[tid] = keep;
+        __syncthreads();
+        for (int sh = 1; sh < BLK; sh <<= 1) {
+            int x = tid >= sh ? s_scan[tid - sh] : 0;
+            __syncthreads();
+            s_scan[tid] += x;
+            __syncthreads();
+        }
+        int nw = kept + s_scan[tid] - keep;
+        int total = s_scan[BLK - 1];
+        __syncthreads();
+        if (keep) {
+            int ci = ctb + nw;
+            D.ct_i1[ci] = i1; D.ct_i2[ci] = i2; D.ct_type[ci] = ty; D.ct_status[ci] = st;
+            D.ct_ref[ci] = rf; D.ct_d1[ci] = a; D.ct_d2[ci] = b; D.ct_vac[ci] = 0;
+            D.ct_F[ci] = fx; D.ct_F[tot + ci] = fy; D.ct_F[2 * tot + ci] = fz; D.ct_V[ci] = v;
+            // remember old -> new in rm_idx's space (old index slot)
+            D.rm_idx[ctb + o] = ci;
+        }
+        kept += total;
+        __syncthreads();
+    }
+    // remap bead slots
+    int s0 = D.rep_slot_off[rep], s1 = s0 + D.rep_n[rep];
+    for (int s = s0 + tid; s < s1; s += BLK) {
+        int q = D.qn[s];
+        for (int k = 0; k < q; ++k) {
+            int old = D.qslot[k * D.nslot + s];
+            D.qslot[k * D.nslot + s] = D.rm_idx[old];
+        }
+    }
+    __syncthreads();
+    for (int o = kept + tid; o < cnt; o += BLK) D.ct_vac[ctb + o] = 1;
+    if (tid == 0) D.ct_count[rep] = kept;
+}
+
+// Langevin predictor-corrector of one bead, xmd/src/dynamics/lang_pc.cpp:11-37,
+// plus the displacement test of nl/verify.cpp:10-15 (max over the replica).
+__device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p, int s, int rep, bool active, double Fx,
+                                           double Fy, double Fz) {
+    const int n = D.nslot;
+    int local = s - D.rep_slot_off[rep];
+    uint64_t S = wave_lane_state(D, D.rng[p][rep], local >> 6);
+    float nz[3];
+    bead_noise(S, D.bm_fix, nz);
+    double disp = 0.0;
+    if (active) {
+        double *y = D.y;
+        v3<double> y0 = ld3(D.pos[p], n, s);
+        v3<double> y1 = ld3(y, n, s), y2 = ld3(y + 3 * n, n, s), y3 = ld3(y + 6 * n, n, s);
+        v3<double> y4 = ld3(y + 9 * n, n, s), y5 = ld3(y + 12 * n, n, s);
+        if (G.lang_on) {
+            double nsd = D.noise_sd[rep];
+            v3<double> v = y1 * G.dt_inv;
+            v3<double> a;
+            a.x = Fx * 1.0 - G.gamma * v.x + nsd * (double)nz[0];
+            a.y = Fy * 1.0 - G.gamma * v.y + nsd * (double)nz[1];
+            a.z = Fz * 1.0 - G.gamma * v.z + nsd * (double)nz[2];
+            v3<double> err = y2 - a * G.half_dt2;
+            y0 = y0 - (3.0 / 16.0) * err;
+            y1 = y1 - (251.0 / 360.0) * err;
+            y2 = y2 - 1.0 * err;
+            y3 = y3 - (11.0 / 18.0) * err;
+            y4 = y4 - (1.0 / 6.0) * err;
+            y5 = y5 - (1.0 / 60.0) * err;
+            y0 = y0 + ((((y1 + y2) + y3) + y4) + y5);
+            y1 = y1 + (((2.0 * y2 + 3.0 * y3) + 4.0 * y4) + 5.0 * y5);
+            y2 = y2 + ((3.0 * y3 + 6.0 * y4) + 10.0 * y5);
+            y3 = y3 + (4.0 * y4 + 10.0 * y5);
+            y4 = y4 + 5.0 * y5;
+        }
+        st3(D.pos[1 - p], n, s, y0);
+        st3(y, n, s, y1);
+        st3(y + 3 * n, n, s, y2);
+        st3(y + 6 * n, n, s, y3);
+        st3(y + 9 * n, n, s, y4);
+        st3(y + 12 * n, n, s, y5);
+        v3<double> dr = r_uv(D.box[rep], y0, ld3(D.orig_r, n, s));
+        disp = norm(dr);
+        if (!(disp == disp) || !(fabs(y0.x) < 1e300)) disp = __builtin_nan("");
+    }
+    // wave max (non-negative doubles order like their bit patterns; NaN wins)
+    unsigned long long bits = active ? (unsigned long long)xmd_bm::dbits(disp) : 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long other = __shfl_xor(bits, o);
+        bits = other > bits ? other : bits;
+    }
+    if (lane_id() == 0 && bits) atomicMax(D.maxd[p] + rep, bits);
+}
+
+// ------------------------------------------------------------- forces ---
+// Force on slot s from every enabled term; FUSED then integrates s.
+template <typename T> struct TermTab {
+    TermParams<T> P;
+    uint32_t mask;
+};
+
+template <typename T>
+__device__ inline void add3(double &x, double &y, double &z, v3<T> f) {
+    x += (double)f.x;
+    y += (double)f.y;
+    z += (double)f.z;
+}
+
+template <typename T, bool FUSED, bool EVAL>
+__global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
+    const int n = D.nslot;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    uint8_t bf = D.bflags[s];
+    bool active = bf & B_ACTIVE;
+    const double *Pp = D.pos[p];
+    const TermParams<T> &P = TT.P;
+    uint32_t mask = TT.mask;
+    double Fx = 0, Fy = 0, Fz = 0, V = 0;
+    double Vt[EVAL ? NUM_TERMS : 1];
+    for (int k = 0; k < (EVAL ? NUM_TERMS : 1); ++k) Vt[k] = 0;
+    auto addV = [&](int term, double v) {
+        V += v;
+        if constexpr (EVAL) Vt[term] += v;
+    };
+    double tpx = 0, tpy = 0, tpz = 0, tnx = 0, tny = 0, tnz = 0;  // PID spill to s-1, s+1
+    v3<double> ri = {0, 0, 0};
+    if (active) {
+        ri = ld3(Pp, n, s);
+        Box box = D.box[rep];
+        int ch = D.chain[s];
+        // chain window s-3 .. s+3
+        v3<double> rw[7];
+        bool inw[7];
+        for (int k = 0; k < 7; ++k) {
+            int a = s + k - 3;
+            inw[k] = a >= 0 && a < n && D.chain[a] == ch;
+            rw[k] = inw[k] ? ld3(Pp, n, a) : ri;
+        }
+        // tether (i, i+1)
+        if (mask & (1u << T_TETHER)) {
+            v3<T> out[2];
+            if (inw[2]) {
+                double nat = D.tether_nat[s - 1];
+                tether<T>(cvt<T>(rw[3] - rw[2]), P.H1, P.H2, (T)nat, out);
+                add3(Fx, Fy, Fz, out[1]);
+            }
+            if (inw[4]) {
+                double nat = D.tether_nat[s];
+                T v = tether<T>(cvt<T>(rw[4] - rw[3]), P.H1, P.H2, (T)nat, out);
+                add3(Fx, Fy, Fz, out[0]);
+                addV(T_TETHER, (double)v);
+            }
+        }
+        // angles (a, a+1, a+2), a = s-2 .. s
+        if (mask & (1u << T_NAT_ANG)) {
+            for (int a = 0; a < 3; ++a) {
+                int k0 = a + 1;  // window index of the first bead (s-2 -> 1)
+                if (!inw[k0] || !inw[k0 + 2]) continue;
+                double th = D.theta_nat[s - 2 + a];
+                if (th != th) continue;
+                v3<T> out[3];
+                T v = nat_angle<T>(cvt<T>(rw[k0 + 1] - rw[k0]), cvt<T>(rw[k0 + 2] - rw[k0 + 1]), P.ang_k, (T)th, out);
+                add3(Fx, Fy, Fz, out[2 - a]);
+                if (a == 2) addV(T_NAT_ANG, (double)v);
+            }
+        }
+        // quads (a .. a+3), a = s-3 .. s: chirality and native dihedrals
+        uint32_t qmask = mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND));
+        if (qmask) {
+            for (int a = 0; a < 4; ++a) {
+                if (!inw[a] || !inw[a + 3]) continue;
+                int first = s - 3 + a;
+                v3<T> r12 = cvt<T>(rw[a + 1] - rw[a]), r23 = cvt<T>(rw[a + 2] - rw[a + 1]),
+                      r34 = cvt<T>(rw[a + 3] - rw[a + 2]);
+                v3<T> out[4];
+                if (mask & (1u << T_CHIRAL)) {
+                    double nc = D.chir_nat[first];
+                    if (nc == nc) {
+                        T v = chiral<T>(r12, r23, r34, P.e_chi, (T)nc, (T)D.chir_fac[first], out);
+                        add3(Fx, Fy, Fz, out[3 - a]);
+                        if (a == 3) addV(T_CHIRAL, (double)v);
+                    }
+                }
+                double ph = D.phi_nat[first];
+                if (ph == ph) {
+                    if (mask & (1u << T_CND)) {
+                        T v = nat_dihedral<T>(r12, r23, r34, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
+                        add3(Fx, Fy, Fz, out[3 - a]);
+                        if (a == 3) addV(T_CND, (double)v);
+                    }
+                    if (mask & (1u << T_SND)) {
+                        T v = nat_dihedral<T>(r12, r23, r34, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
+                        add3(Fx, Fy, Fz, out[3 - a]);
+                        if (a == 3) addV(T_SND, (double)v);
+                    }
+                }
+            }
+        }
+        // native contacts (Go)
+        if (mask & (1u << T_GO)) {
+            for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k) {
+                if (!D.nat_act[k]) continue;
+                int j = D.nat_j[k];
+                v3<double> d = wrap(box, ld3(Pp, n, j) - ri);
+                FV<T> f = go_pair<T>(cvt<T>(d), P.go_depth, (T)D.nat_d[k]);
+                add3(Fx, Fy, Fz, f.F);
+                if (s < j) addV(T_GO, (double)f.V);
+            }
+        }
+        // neighbour row: Pauli, Debye-Hueckel, PID
+        uint32_t rmask = mask & ((1u << T_PAULI) | (1u << T_CONST_ES) | (1u << T_REL_ES) | (1u << T_PID));
+        if (rmask && !(D.err[0] & E_NL_CAP)) {
+            int len = D.nl_len[s];
+            int base = D.slice_start[s >> 6] + (s & 63);
+            int es_term = P.es_relative ? T_REL_ES : T_CONST_ES;
+            bool es_on = mask & (1u << es_term);
+            T q_s = (T)D.charge[D.atype[s]];
+            for (int k = 0; k < len; ++k) {
+                int e = base + 64 * k;
+                uint8_t fl = D.nl_flag[e];
+                if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
+                int j = D.nl_j[e];
+                v3<double> rj = ld3(Pp, n, j);
+                v3<double> dd = wrap(box, rj - ri);
+                v3<T> d = cvt<T>(dd);
+                bool owner = s < j;
+                if ((fl & F_PAULI) && (mask & (1u << T_PAULI))) {
+                    FV<T> f = pauli_pair<T>(d, P.pauli_depth, P.pauli_r_excl);
+                    add3(Fx, Fy, Fz, f.F);
+                    if (owner) addV(T_PAULI, (double)f.V);
+                }
+                if ((fl & F_ES) && es_on) {
+                    T q1q2 = owner ? q_s * (T)D.charge[D.atype[j]] : (T)D.charge[D.atype[j]] * q_s;
+                    FV<T> f = es_pair<T>(d, q1q2, P.es_Vf, P.es_sdi, P.es_relative);
+                    add3(Fx, Fy, Fz, f.F);
+                    if (owner) addV(es_term, (double)f.V);
+                }
+                if ((fl & F_PID) && (mask & (1u << T_PID))) {
+                    int i1 = owner ? s : j, i2 = owner ? j : s;
+                    v3<double> p1 = owner ? ri : rj, p2 = owner ? rj : ri;
+                    v3<double> p1p = ld3(Pp, n, i1 - 1), p1n = ld3(Pp, n, i1 + 1);
+                    v3<double> p2p = ld3(Pp, n, i2 - 1), p2n = ld3(Pp, n, i2 + 1);
+                    v3<double> r12 = owner ? dd : -dd;
+                    int ty = 20 * D.atype[i1] + D.atype[i2];
+                    v3<T> out[6];
+                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], cvt<T>(p1 - p1p),
+                                        cvt<T>(p1n - p1), cvt<T>(p1n - p1p), cvt<T>(r12), cvt<T>(p2 - p2p),
+                                        cvt<T>(p2n - p2), cvt<T>(p2n - p2p), cvt<T>(-r12), cvt<T>(r12), out);
+                    int o = owner ? 0 : 3;
+                    add3(Fx, Fy, Fz, out[o + 1]);
+                    add3(tpx, tpy, tpz, out[o]);
+                    add3(tnx, tny, tnz, out[o + 2]);
+                    if (owner) addV(T_PID, (double)v);
+                }
+            }
+        }
+        // quasi-adiabatic contacts through the bead's contact slots
+        if (mask & (1u << T_QA)) {
+            int q = D.qn[s], tot = D.ct_cap[D.nrep];
+            for (int k = 0; k < q; ++k) {
+                int ci = D.qslot[k * n + s];
+                double fx = D.ct_F[ci], fy = D.ct_F[tot + ci], fz = D.ct_F[2 * tot + ci];
+                if (D.ct_i1[ci] == s) {
+                    Fx += fx;
+                    Fy += fy;
+                    Fz += fz;
+                    addV(T_QA, D.ct_V[ci]);
+                } else {
+                    Fx -= fx;
+                    Fy -= fy;
+                    Fz -= fz;
+                }
+            }
+        }
+    }
+    if constexpr (EVAL) {
+        if (active)
+            for (int k = 0; k < NUM_TERMS; ++k) D.Vterm[k * n + s] = Vt[k];
+    }
+    if constexpr (!FUSED) {
+        if (active) {
+            D.Fbuf[s] = Fx;
+            D.Fbuf[n + s] = Fy;
+            D.Fbuf[2 * n + s] = Fz;
+            D.Vbead[s] = V;
+            double *ps = D.pid_side;
+            ps[s] = tpx; ps[n + s] = tpy; ps[2 * n + s] = tpz;
+            ps[3 * n + s] = tnx; ps[4 * n + s] = tny; ps[5 * n + s] = tnz;
+        }
+        return;
+    } else {
+        if (active) D.Vbead[s] = V;
+        // integrate (whole wavefront participates in the RNG jump)
+        integrate_bead_impl(D, G, p, s, rep, active, Fx, Fy, Fz);
+    }
+}
+
+// Integration when PID spilled forces onto chain neighbours.
+__global__ __launch_bounds__(BLK) void k_integrate(Dev D, Derived G, int p) {
+    const int n = D.nslot;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    uint8_t bf = D.bflags[s];
+    bool active = bf & B_ACTIVE;
+    double Fx = 0, Fy = 0, Fz = 0;
+    if (active) {
+        Fx = D.Fbuf[s];
+        Fy = D.Fbuf[n + s];
+        Fz = D.Fbuf[2 * n + s];
+        const double *ps = D.pid_side;
+        if (bf & B_PREV) {
+            Fx += ps[3 * n + s - 1];
+            Fy += ps[4 * n + s - 1];
+            Fz += ps[5 * n + s - 1];
+        }
+        if (bf & B_NEXT) {
+            Fx += ps[s + 1];
+            Fy += ps[n + s + 1];
+            Fz += ps[2 * n + s + 1];
+        }
+    }
+    integrate_bead_impl(D, G, p, s, rep, active, Fx, Fy, Fz);
+}
+
+// Total force of the non-fused path (eval_forces): F += PID spill.
+__global__ __launch_bounds__(BLK) void k_gather_force(Dev D) {
+    const int n = D.nslot;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    uint8_t bf = D.bflags[s];
+    if (!(bf & B_ACTIVE)) return;
+    const double *ps = D.pid_side;
+    double Fx = D.Fbuf[s], Fy = D.Fbuf[n + s], Fz = D.Fbuf[2 * n + s];
+    if (bf & B_PREV) {
+        Fx += ps[3 * n + s - 1];
+        Fy += ps[4 * n + s - 1];
+        Fz += ps[5 * n + s - 1];
+    }
+    if (bf & B_NEXT) {
+        Fx += ps[s + 1];
+        Fy += ps[n + s + 1];
+        Fz += ps[2 * n + s + 1];
+    }
+    D.Fbuf[s] = Fx;
+    D.Fbuf[n + s] = Fy;
+    D.Fbuf[2 * n + s] = Fz;
+}
+
+// One wavefront per replica: advance rand_gen by 4N draws and the time by dt
+// (lang_pc.cpp:39-40), then decide the Verlet rebuild (verify.cpp:17-21).
+// mode 0: decision from force_rb only (no advance).
+__global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int mode) {
+    int rep = blockIdx.x;
+    int l = lane_id();
+    if (rep == 0 && l == 0) D.any_rb[1 - p] = 0;
+    if (mode == 1) {
+        int nb = D.rep_n[rep];
+        uint64_t S = wave_lane_state(D, D.rng[p][rep], nb >> 6);
+        uint64_t Sn = __shfl(S, nb & 63);
+        if (l == 0) {
+            D.rng[1 - p][rep] = Sn;
+            D.tt[1 - p][rep] = D.tt[p][rep] + G.dt;
+        }
+    }
+    if (l == 0) {
+        int rb = D.force_rb[rep];
+        if (mode == 1) {
+            double m = xmd_bm::bitsd(D.maxd[p][rep]);
+            if (!(m == m)) atomicOr(D.err, E_NONFINITE);
+            double total = 2.0 * m + 0.0;
+            if (total > G.pad) rb = 1;
+            D.maxd[1 - p][rep] = 0ull;
+        } else {
+            D.maxd[p][rep] = 0ull;
+            D.maxd[1 - p][rep] = 0ull;
+        }
+        D.rb_now[rep] = rb;
+        D.force_rb[rep] = 0;
+        if (rb) atomicOr(D.any_rb + p, 1);
+    }
+}
+
+// ------------------------------------------------------------- rebuild ---
+__device__ inline int cell_coord(double x, double L, int nc) {
+    double w = x - L * floor(x / L);
+    int c = (int)(w / L * (double)nc);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ __launch_bounds__(BLK) void k_cell_count(Dev D, const double *P, int p) {
+    if (!D.any_rb[p]) return;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    if (!(D.bflags[s] & B_ACTIVE) || !D.rb_now[rep]) return;
+    int c0 = D.cell_off[rep];
+    if (D.cell_off[rep + 1] == c0) return;
+    const int *nd = D.cell_dim + 3 * rep;
+    Box b = D.box[rep];
+    int n = D.nslot;
+    int cx = cell_coord(P[s], b.cx, nd[0]), cy = cell_coord(P[n + s], b.cy, nd[1]),
+        cz = cell_coord(P[2 * n + s], b.cz, nd[2]);
+    int c = c0 + (cz * nd[1] + cy) * nd[0] + cx;
+    D.bead_cell[s] = c;
+    atomicAdd(D.cell_cnt + c, 1);
+}
+
+// One workgroup per replica: exclusive scan of a per-replica segment.
+// mode 0: cells (cell_cnt -> cell_start/end/cursor, absolute slot offsets);
+// mode 1: row slices (64*slice_w -> slice_start, capacity check);
+// mode 2: free-pair slices (fp_slice_cnt -> fp_slice_start, fp_count).
+__global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
+    __shared__ int s_buf[1024];
+    __shared__ int s_carry;
+    if (!D.any_rb[p]) return;
+    int rep = blockIdx.x, tid = threadIdx.x;
+    if (!D.rb_now[rep]) return;
+    int lo, hi, base;
+    if (mode == 0) {
+        lo = D.cell_off[rep];
+        hi = D.cell_off[rep + 1];
+        base = D.rep_slot_off[rep];
+    } else {
+        lo = D.rep_slot_off[rep] >> 6;
+        hi = (D.rep_slot_off[rep] + D.rep_n[rep] + 63) >> 6;
+        base = mode == 1 ? D.nl_base[rep] : D.fp_base[rep];
+    }
+    if (tid == 0) s_carry = base;
+    __syncthreads();
+    for (int c0 = lo; c0 < hi; c0 += 1024) {
+        int c = c0 + tid;
+        int v = 0;
+        if (c < hi) v = mode == 0 ? D.cell_cnt[c] : (mode == 1 ? 64 * D.slice_w[c] : D.fp_slice_cnt[c]);
+        s_buf[tid] = v;
+        __syncthreads();
+        for (int sh = 1; sh < 1024; sh <<= 1) {
+            int x = tid >= sh ? s_buf[tid - sh] : 0;
+            __syncthreads();
+            s_buf[tid] += x;
+            __syncthreads();
+        }
+        int ex = s_carry + s_buf[tid] - v;
+        if (c < hi) {
+            if (mode == 0) {
+                D.cell_start[c] = ex;
+                D.cell_end[c] = ex + v;
+                D.cell_cursor[c] = ex;
+            } else if (mode == 1) {
+                D.slice_start[c] = ex;
+            } else {
+                D.fp_slice_start[c] = ex;
+            }
+        }
+        __syncthreads();
+        if (tid == 1023) s_carry = ex + v;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int used = s_carry - base;
+        if (mode == 1 && used > D.nl_cap[rep]) atomicOr(D.err, E_NL_CAP);
+        if (mode == 2) {
+            if (used > D.fp_cap[rep]) atomicOr(D.err, E_FP_CAP);
+            D.fp_count[rep] = used > D.fp_cap[rep] ? D.fp_cap[rep] : used;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_cell_fill(Dev D, int p) {
+    if (!D.any_rb[p]) return;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    if (!(D.bflags[s] & B_ACTIVE) || !D.rb_now[rep]) return;
+    if (D.cell_off[rep + 1] == D.cell_off[rep]) return;
+    int c = D.bead_cell[s];
+    int k = atomicAdd(D.cell_cursor + c, 1);
+    D.cell_beads[k] = s;
+}
+
+// Visit every candidate partner j of slot s (cells or the whole replica).
+template <typename Fn> __device__ inline void for_candidates(const Dev &D, int s, int rep, Fn fn) {
+    int c0 = D.cell_off[rep];
+    if (D.cell_off[rep + 1] == c0) {
+        int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
+        for (int j = a; j < b; ++j)
+            if (j != s) fn(j);
+        return;
+    }
+    const int *nd = D.cell_dim + 3 * rep;
+    int c = D.bead_cell[s] - c0;
+    int cx = c % nd[0], cy = (c / nd[0]) % nd[1], cz = c / (nd[0] * nd[1]);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = (cx + dx + nd[0]) % nd[0], y = (cy + dy + nd[1]) % nd[1], z = (cz + dz + nd[2]) % nd[2];
+                int cc = c0 + (z * nd[1] + y) * nd[0] + x;
+                for (int k = D.cell_start[cc]; k < D.cell_end[cc]; ++k) {
+                    int j = D.cell_beads[k];
+                    if (j != s) fn(j);
+                }
+            }
+}
+
+// Row flags of the pair (s, j) at rebuild: legacy_update's criterion
+// (legacy_update.cpp:24-29) and the per-term filters of update_X
+// (pauli.cpp:13-20, es_pair.cpp:9-21, bundle.cpp:9-26, free_pair.cpp:9-19).
+__device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &box, int s, int j, v3<double> rs,
+                                     const double *P) {
+    if (D.chain[j] == D.chain[s] && (s > j ? s - j : j - s) < 3) return 0;
+    v3<double> d = wrap(box, ld3(P, D.nslot, j) - rs);
+    double dist = norm(d);
+    if (!(dist < G.req_r)) return 0;
+    // native?  rows are short and sorted
+    bool native = false;
+    for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k)
+        if (D.nat_j[k] == j) {
+            native = true;
+            break;
+        }
+    double ninv = 1.0 / dist;
+    uint8_t f = native ? F_NATIVE : 0;
+    if (G.pauli_on && !native && ninv > G.pauli_min_inv) f |= F_PAULI;
+    if (G.es_on && ninv > G.es_min_inv && D.charge[D.atype[s]] * D.charge[D.atype[j]] != 0.0) f |= F_ES;
+    if (G.pid_on && !native && ninv > G.pid_min_inv) {
+        uint8_t a = D.bflags[s], b = D.bflags[j];
+        if ((a & B_PREV) && (a & B_NEXT) && (b & B_PREV) && (b & B_NEXT)) f |= F_PID;
+    }
+    if (G.qa_on && !native && s < j && ninv > G.qa_min_inv) {
+        bool in_contact = false;
+        int q = D.qn[s];
+        int tot = D.ct_cap[D.nrep];
+        (void)tot;
+        for (int k = 0; k < q; ++k) {
+            int ci = D.qslot[k * D.nslot + s];
+            if (D.ct_i1[ci] == s ? D.ct_i2[ci] == j : D.ct_i1[ci] == j) in_contact = true;
+        }
+        if (!in_contact) f |= F_QAFREE;
+    }
+    return f;
+}
+
+__global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double *P, int p) {
+    if (!D.any_rb[p]) return;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    if (!D.rb_now[rep]) return;  // wave-uniform: waves never span replicas
+    int cnt = 0;
+    if (D.bflags[s] & B_ACTIVE) {
+        Box box = D.box[rep];
+        v3<double> rs = ld3(P, D.nslot, s);
+        for_candidates(D, s, rep, [&](int j) {
+            uint8_t f = pair_flags(D, G, box, s, j, rs, P);
+            if (f & (F_PAULI | F_ES | F_PID | F_QAFREE)) ++cnt;
+        });
+    }
+    D.nl_len[s] = cnt;
+    int m = cnt;
+    for (int o = 32; o > 0; o >>= 1) {
+        int x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+    }
+    if (lane_id() == 0) D.slice_w[s >> 6] = m;
+}
+
+__global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double *P, int p) {
+    if (!D.any_rb[p]) return;
+    if (*D.err & E_NL_CAP) return;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    if (!D.rb_now[rep]) return;
+    const int n = D.nslot;
+    int nfp = 0;
+    if (D.bflags[s] & B_ACTIVE) {
+        Box box = D.box[rep];
+        v3<double> rs = ld3(P, n, s);
+        int base = D.slice_start[s >> 6] + (s & 63);
+        int cnt = 0;
+        for_candidates(D, s, rep, [&](int j) {
+            uint8_t f = pair_flags(D, G, box, s, j, rs, P);
+            if (f & (F_PAULI | F_ES | F_PID | F_QAFREE)) {
+                // insertion into the sorted row
+                int k = cnt++;
+                while (k > 0 && D.nl_j[base + 64 * (k - 1)] > j) {
+                    D.nl_j[base + 64 * k] = D.nl_j[base + 64 * (k - 1)];
+                    D.nl_flag[base + 64 * k] = D.nl_flag[base + 64 * (k - 1)];
+                    --k;
+                }
+                D.nl_j[base + 64 * k] = j;
+                D.nl_flag[base + 64 * k] = f;
+            }
+        });
+        int half = 0;
+        for (int k = 0; k < cnt; ++k) {
+            int j = D.nl_j[base + 64 * k];
+            if (j < s) half = k + 1;
+            if (D.nl_flag[base + 64 * k] & F_QAFREE) ++nfp;
+        }
+        D.nl_half[s] = half;
+        // snapshot for verify, Go activity (go.cpp:11-22)
+        st3(D.orig_r, n, s, rs);
+        for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k) {
+            int j = D.nat_j[k];
+            v3<double> d = wrap(box, ld3(P, n, j) - rs);
+            double cutoff = 2.0 * D.nat_d[k];
+            D.nat_act[k] = norm(d) < cutoff + G.pad;
+        }
+    }
+    int sum = nfp;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (lane_id() == 0) D.fp_slice_cnt[s >> 6] = sum;
+}
+
+// Free pairs in canonical order; zero the cell counters; count the rebuild.
+__global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
+    if (!D.any_rb[p]) return;
+    int s = blockIdx.x * BLK + threadIdx.x;
+    int rep = D.slot_rep[s];
+    if (!D.rb_now[rep]) return;
+    const int n = D.nslot;
+    int nfp = 0;
+    bool active = D.bflags[s] & B_ACTIVE;
+    int base = D.slice_start[s >> 6] + (s & 63);
+    int len = active ? D.nl_len[s] : 0;
+    for (int k = 0; k < len; ++k) nfp += (D.nl_flag[base + 64 * k] & F_QAFREE) != 0;
+    // exclusive prefix within the wavefront
+    int incl = nfp;
+    for (int o = 1; o < 64; o <<= 1) {
+        int x = __shfl_up(incl, o);
+        if (lane_id() >= o) incl += x;
+    }
+    int off = D.fp_slice_start[s >> 6] + incl - nfp;
+    int end = D.fp_base[rep] + D.fp_cap[rep];
+    for (int k = 0; k < len; ++k) {
+        int e = base + 64 * k;
+        if (D.nl_flag[e] & F_QAFREE) {
+            if (off < end) {
+                D.fp_i1[off] = s;
+                D.fp_i2[off] = D.nl_j[e];
+                D.fp_vac[off] = 0;
+            }
+            ++off;
+        }
+    }
+    int local = s - D.rep_slot_off[rep];
+    int c0 = D.cell_off[rep], nc = D.cell_off[rep + 1] - c0;
+    for (int c = local; c < nc; c += D.rep_n[rep]) D.cell_cnt[c0 + c] = 0;
+    if (local == 0) D.n_rebuild[rep] += 1;
+    (void)n;
+}
+
+// Per-bead count / fill of the full canonical Verlet list from orig_r
+// (export only, legacy_update.cpp:16-62).  mode 0 counts, 1 fills.
+__global__ __launch_bounds__(BLK) void k_nl_export(Dev D, Derived G, int *cnt, const int *off, int *oi1, int *oi2,
+                                                   uint8_t *onat, int mode) {
+    int s = blockIdx.x * BLK + threadIdx.x;
+    if (!(D.bflags[s] & B_ACTIVE)) return;
+    int rep = D.slot_rep[s];
+    const int n = D.nslot;
+    Box box = D.box[rep];
+    v3<double> rs = ld3(D.orig_r, n, s);
+    int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
+    int k = mode ? off[s] : 0;
+    for (int j = s + 1; j < b; ++j) {
+        if (D.chain[j] == D.chain[s] && j - s < 3) continue;
+        v3<double> d = wrap(box, ld3(D.orig_r, n, j) - rs);
+        if (norm(d) < G.req_r) {
+            if (mode) {
+                bool native = false;
+                for (int q = D.nat_start[s]; q < D.nat_start[s + 1]; ++q)
+                    if (D.nat_j[q] == j) native = true;
+                oi1[k] = s;
+                oi2[k] = j;
+                onat[k] = native;
+            }
+            ++k;
+        }
+    }
+    if (!mode) cnt[s] = k;
+    (void)a;
+}
+
+// rand_gen noise of n consecutive beads of one stream (xmd_rand_normals).
+__global__ __launch_bounds__(BLK) void k_rand_normals(Dev D, uint64_t S, long long nb, float *out) {
+    long long k = (long long)blockIdx.x * BLK + threadIdx.x;
+    int w = (int)(k >> 6);
+    uint64_t st = wave_lane_state(D, S, w);
+    float nz[3];
+    bead_noise(st, D.bm_fix, nz);
+    if (k < nb) {
+        out[3 * k] = nz[0];
+        out[3 * k + 1] = nz[1];
+        out[3 * k + 2] = nz[2];
+    }
+}
+
+}  // namespace xmd_k

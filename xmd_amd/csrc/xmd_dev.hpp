@@ -1,0 +1,457 @@
+// xmd_dev.hpp -- per-item physics of the step path as __host__ __device__
+// functions, templated on the force precision T (double, or float in the
+// mixed-precision form).  Positions and displacements are always double;
+// mixed kernels narrow FP64-differenced displacements to float.
+//
+// Every function restates one reference functor body (file:line at each),
+// with the engine's decisions D1-D19 (listed in xmd_oracle.cpp's header).
+// Expression order follows the reference; the file is compiled with
+// -ffp-contract=off so the FP64 decision code (lists, QA) evaluates exactly
+// as written.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#include "bm_math.hpp"
+
+namespace xmd_dev {
+
+template <typename T> struct v3 {
+    T x, y, z;
+};
+template <typename T> XMD_HD inline v3<T> mk3(T x, T y, T z) { return v3<T>{x, y, z}; }
+template <typename T> XMD_HD inline v3<T> operator+(v3<T> a, v3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename T> XMD_HD inline v3<T> operator-(v3<T> a, v3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T> XMD_HD inline v3<T> operator-(v3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <typename T> XMD_HD inline v3<T> operator*(v3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename T> XMD_HD inline v3<T> operator*(T s, v3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> XMD_HD inline v3<T> operator/(v3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <typename T> XMD_HD inline T dot(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> XMD_HD inline T norm_sq(v3<T> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+template <typename T> XMD_HD inline T norm(v3<T> a) { return sqrt(norm_sq(a)); }
+template <typename T> XMD_HD inline T norm_inv(v3<T> a) { return T(1.0) / norm(a); }
+template <typename T> XMD_HD inline v3<T> cross(v3<T> a, v3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename T> XMD_HD inline v3<T> unit(v3<T> a) { return norm_inv(a) * a; }
+template <typename T> XMD_HD inline T clampT(T v, T lo, T hi) { return v < lo ? lo : (v < hi ? v : hi); }
+template <typename T> XMD_HD inline T ipow6(T s) { return s * (s * (s * (s * (s * s)))); }  // math.h:15-19
+template <typename T> XMD_HD inline T acos_c(T x) {  // xmd/include/xmd/utils/math.h:41-45
+    T x_ = x < T(-1.0) ? T(-1.0) : (x > T(1.0) ? T(1.0) : x);
+    return acos(x_);
+}
+template <typename T, typename U> XMD_HD inline v3<T> cvt(v3<U> a) { return {(T)a.x, (T)a.y, (T)a.z}; }
+
+// Orthorhombic box, xmd/include/xmd/model/box.h:36-49 (cell_inv 0 = open).
+struct Box {
+    double cx, cy, cz, ix, iy, iz;
+};
+XMD_HD inline v3<double> wrap(const Box &b, v3<double> v) {
+    v3<double> o = v;
+    o.x -= b.cx * rint(v.x * b.ix);
+    o.y -= b.cy * rint(v.y * b.iy);
+    o.z -= b.cz * rint(v.z * b.iz);
+    return o;
+}
+XMD_HD inline v3<double> r_uv(const Box &b, v3<double> u, v3<double> v) { return wrap(b, v - u); }
+
+// --------------------------------------------------------- primitives ---
+template <typename T> struct VdV {
+    T V, dV;
+};
+template <typename T> XMD_HD inline VdV<T> lj(T depth, T r_min, T r_inv) {  // lj.h:22-29
+    T s = r_inv * r_min, s6 = ipow6(s), s12 = s6 * s6;
+    T V = depth * (s12 - T(2.0) * s6);
+    T dV = T(12.0) * depth * r_inv * (s6 - s12);
+    return {clampT(V, T(-1.0e3), T(1.0e3)), clampT(dV, T(-1.0e3), T(1.0e3))};
+}
+template <typename T> XMD_HD inline VdV<T> shifted_lj(T depth, T r_min, T r_inv) {  // shifted_lj.h:14-21
+    T s = r_inv * r_min, s6 = ipow6(s), s12 = s6 * s6;
+    T V = depth * (s12 - T(2.0) * s6 + T(1.0));
+    T dV = T(12.0) * depth * r_inv * (s6 - s12);
+    return {clampT(V, T(-1.0e3), T(1.0e3)), clampT(dV, T(-1.0e3), T(1.0e3))};
+}
+template <typename T> XMD_HD inline VdV<T> sink_lj(T depth, T r_min, T r_max, T r, T r_inv) {  // sink_lj.h:24-33
+    T r_eff = (r < r_min ? r_min : (r < r_max ? r : r_max));
+    T s = r_inv * r_eff, s6 = ipow6(s), s12 = s6 * s6;
+    T V = depth * (s12 - T(2.0f) * s6);
+    T dV = T(12.0f) * depth * r_inv * (s6 - s12);
+    dV = (r_min < r && r < r_max) ? T(0) : dV;
+    return {clampT(V, T(-1.0e3), T(1.0e3)), clampT(dV, T(-1.0e3), T(1.0e3))};
+}
+template <typename T> XMD_HD inline VdV<T> harmonic(T H1, T H2, T nat_r, T r) {  // harmonic.h:18-25
+    T dr = r - nat_r, dr2 = dr * dr;
+    T V = T(0.5) * dr2 * (H1 + H2 * dr2);
+    T dV = dr * (H1 + T(2.0) * H2 * dr2);
+    return {clampT(V, T(-1.0e3), T(1.0e3)), clampT(dV, T(-1.0e3), T(1.0e3))};
+}
+// lambda.h:30-56 (+D19)
+template <typename T> XMD_HD inline bool lam_supp(T psi0, T alpha, T psi) {
+    return fabs(alpha * (psi - psi0)) < T(M_PI);
+}
+template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, int version, T psi) {
+    T s = alpha * (psi - psi0);
+    if (version == 0) {
+        T val = T(0.5f) * cos(s) + T(0.5f);
+        T deriv = T(-0.5f) * alpha * sin(s);
+        return {val, deriv};
+    }
+    T t = fabs(s / T(M_PI));
+    T x_inv = T(1.0f) / (T(2.0f) * t * t - T(2.0f) * t - T(1));
+    T val = (t * t - T(2.0f) * t + T(1.0f)) * x_inv;
+    T deriv = T(2.0f) * (t - T(1.0f)) * (t - T(2.0f)) * x_inv * x_inv * (alpha / T(M_PI));
+    deriv *= (s < T(0.0f) ? T(-1.0f) : T(1.0f));
+    return {val, deriv};
+}
+
+// ------------------------------------------------------------- params ---
+// Scalar parameters in the kernels' precision T (narrowed from double on
+// the host, like `real` fields set in xmd/src/freeform/main.cpp:345-907).
+template <typename T> struct TermParams {
+    T H1, H2, e_chi, ang_k, CDA, CDB, CDH;
+    T pauli_r_excl, pauli_depth, go_depth;
+    T es_Vf, es_sdi;
+    int es_relative;
+    T bbp_psi0, bbp_alpha, bbp_rmin, bbp_depth;
+    T bbm_psi0, bbm_alpha, bbm_rmin, bbm_depth;
+    T ss_psi0, ss_alpha;
+    int lambda_version;
+};
+
+// ------------------------------------------------------- pair terms ---
+// `d` = wrap(r_j - r_g): the displacement from the bead g being summed to
+// its partner j.  The returned force is the force on g, which equals the
+// reference's F[i1] when g = i1 and F[i2] when g = i2 (d is then -r12).
+template <typename T> struct FV {
+    v3<T> F;
+    T V;
+};
+// xmd/src/forces/go.cpp:31-46
+template <typename T> XMD_HD inline FV<T> go_pair(v3<T> d, T depth, T nat_dist) {
+    T rn = norm_inv(d);
+    v3<T> u = d * rn;
+    VdV<T> e = lj(depth, nat_dist, rn);
+    return {u * e.dV, e.V};
+}
+// xmd/src/forces/pauli.cpp:32-48
+template <typename T> XMD_HD inline FV<T> pauli_pair(v3<T> d, T depth, T r_excl) {
+    T rn = norm_inv(d);
+    if (T(1.0f) < rn * r_excl) {
+        v3<T> u = d * rn;
+        VdV<T> e = shifted_lj(depth, r_excl, rn);
+        return {u * e.dV, e.V};
+    }
+    return {{T(0), T(0), T(0)}, T(0)};
+}
+// es/const.cpp:13-32 (+D9) and es/relative.cpp:13-32
+template <typename T> XMD_HD inline FV<T> es_pair(v3<T> d, T q1q2, T Vf, T sdi, int relative) {
+    T r = norm(d), rn = T(1.0f) / r;
+    v3<T> u = d * rn;
+    T V, dV;
+    if (relative) {
+        V = Vf * q1q2 * exp(-r * sdi) * rn * rn;
+        dV = -V * (sdi + T(2.0f) * rn);
+    } else {
+        V = Vf * q1q2 * exp(-r * sdi) * rn;
+        dV = -V * (sdi + rn);
+    }
+    return {u * dV, V};
+}
+
+// One pseudo-improper dihedral (eval_forces.cpp:27-57) with the gradient
+// of psi (D8).  Inputs: r12 = cen - prev, r23 = next - cen, r13 = next -
+// prev, r24 = wrap(partner - cen).
+template <typename T> struct Psi {
+    T psi;
+    v3<T> g_prev, g_cen, g_next, g_par;
+};
+template <typename T> XMD_HD inline Psi<T> pid_psi(v3<T> r12, v3<T> r23, v3<T> r13, v3<T> r24) {
+    v3<T> r14 = r12 + r24;
+    v3<T> rij = -r23, rkj = -r13, rkl = -r14;
+    v3<T> rm = cross(r23, r12);
+    v3<T> rn = cross(r13, r14);
+    T rm_rn = norm_inv(rm), rn_rn = norm_inv(rn);
+    T rkj_n = norm(rkj);
+    T cos_psi = dot(rm, rn) * rm_rn * rn_rn;
+    T psi = acos_c(cos_psi);
+    if (dot(rij, rn) < T(0.0f)) psi = -psi;
+    v3<T> gi = rm * (rkj_n * rm_rn * rm_rn);
+    v3<T> gl = -(rn * (rkj_n * rn_rn * rn_rn));
+    T inv2 = T(1) / (rkj_n * rkj_n);
+    T p = dot(rij, rkj) * inv2, q = dot(rkl, rkj) * inv2;
+    v3<T> gj = (p - T(1)) * gi - q * gl;
+    v3<T> gk = (q - T(1)) * gl - p * gi;
+    return {psi, gk, gi, gj, gl};
+}
+// Full PID bundle (eval_forces.cpp:93-140, +D8).  Geometry of the canonical
+// pair (i1 < i2).  out[0..5] = forces on i1p, i1, i1n, i2p, i2, i2n.
+template <typename T>
+XMD_HD inline T pid_bundle(const TermParams<T> &P, T ss_depth, T ss_rmin, T ss_rmax, v3<T> a12, v3<T> a23, v3<T> a13,
+                           v3<T> a24, v3<T> b12, v3<T> b23, v3<T> b13, v3<T> b24, v3<T> r12, v3<T> out[6]) {
+    Psi<T> g1 = pid_psi(a12, a23, a13, a24);
+    Psi<T> g2 = pid_psi(b12, b23, b13, b24);
+    T psi1 = g1.psi, psi2 = g2.psi;
+    T r12_n = norm(r12), r12_rn = T(1.0f) / r12_n;
+    T A = 0, B = 0, C = 0, V = 0;
+    if (lam_supp(P.bbp_psi0, P.bbp_alpha, psi1) && lam_supp(P.bbp_psi0, P.bbp_alpha, psi2)) {
+        VdV<T> l1 = lam_eval(P.bbp_psi0, P.bbp_alpha, P.lambda_version, psi1);
+        VdV<T> l2 = lam_eval(P.bbp_psi0, P.bbp_alpha, P.lambda_version, psi2);
+        VdV<T> e = lj(P.bbp_depth, P.bbp_rmin, r12_rn);
+        V += l1.V * l2.V * e.V;
+        A += l1.dV * l2.V * e.V;
+        B += l2.dV * l1.V * e.V;
+        C += l1.V * l2.V * e.dV;
+    }
+    if (lam_supp(P.bbm_psi0, P.bbm_alpha, psi1) && lam_supp(P.bbm_psi0, P.bbm_alpha, psi2)) {
+        VdV<T> l1 = lam_eval(P.bbm_psi0, P.bbm_alpha, P.lambda_version, psi1);
+        VdV<T> l2 = lam_eval(P.bbm_psi0, P.bbm_alpha, P.lambda_version, psi2);
+        VdV<T> e = lj(P.bbm_depth, P.bbm_rmin, r12_rn);
+        V += l1.V * l2.V * e.V;
+        A += l1.dV * l2.V * e.V;
+        B += l2.dV * l1.V * e.V;
+        C += l1.V * l2.V * e.dV;
+    }
+    if (lam_supp(P.ss_psi0, P.ss_alpha, psi1) && lam_supp(P.ss_psi0, P.ss_alpha, psi2)) {
+        VdV<T> l1 = lam_eval(P.ss_psi0, P.ss_alpha, P.lambda_version, psi1);
+        VdV<T> l2 = lam_eval(P.ss_psi0, P.ss_alpha, P.lambda_version, psi2);
+        VdV<T> e = sink_lj(ss_depth, ss_rmin, ss_rmax, r12_n, r12_rn);
+        V += l1.V * l2.V * e.V;
+        A += l1.dV * l2.V * e.V;
+        B += l2.dV * l1.V * e.V;
+        C += l1.V * l2.V * e.dV;
+    }
+    v3<T> u = r12 * r12_rn;
+    out[0] = -(A * g1.g_prev);
+    out[1] = -(A * g1.g_cen) - B * g2.g_par + C * u;
+    out[2] = -(A * g1.g_next);
+    out[3] = -(B * g2.g_prev);
+    out[4] = -(A * g1.g_par) - B * g2.g_cen - C * u;
+    out[5] = -(B * g2.g_next);
+    return V;
+}
+
+// ---------------------------------------------------- bonded terms ---
+// Each returns the forces on all of its beads; a bead kernel keeps its own.
+// xmd/src/forces/tether.cpp:21-36 (no wrap)
+template <typename T> XMD_HD inline T tether(v3<T> r12, T H1, T H2, T nat, v3<T> out[2]) {
+    T r12_n = norm(r12);
+    VdV<T> e = harmonic(H1, H2, nat, r12_n);
+    v3<T> u = r12 / r12_n;
+    out[0] = e.dV * u;
+    out[1] = -(e.dV * u);
+    return e.V;
+}
+// xmd/src/forces/chiral.cpp:15-37 (+D16)
+template <typename T> XMD_HD inline T chiral(v3<T> r12, v3<T> r23, v3<T> r34, T e_chi, T nat_chir, T nat_factor,
+                                            v3<T> out[4]) {
+    v3<T> x12_23 = cross(r12, r23), x12_34 = cross(r12, r34), x23_34 = cross(r23, r34);
+    T chir = dot(r12, x23_34) * nat_factor;
+    T chir_diff = chir - nat_chir;
+    T V = T(0.5f) * e_chi * chir_diff * chir_diff;
+    T f = e_chi * chir_diff * nat_factor;
+    out[0] = f * x23_34;
+    out[1] = -(f * (x12_34 + x23_34));
+    out[2] = f * (x12_23 + x12_34);
+    out[3] = -(f * x12_23);
+    return V;
+}
+// xmd/src/forces/angle/native.cpp:13-39
+template <typename T> XMD_HD inline T nat_angle(v3<T> r12, v3<T> r23, T k, T nat_theta, v3<T> out[3]) {
+    v3<T> x12_23 = cross(r12, r23);
+    T r12_rn = norm_inv(r12), r23_rn = norm_inv(r23);
+    v3<T> d1 = unit(cross(r12, x12_23)) * r12_rn;
+    v3<T> d3 = unit(cross(r23, x12_23)) * r23_rn;
+    v3<T> d2 = -d1 - d3;
+    T cos_theta = -dot(r12, r23) * r12_rn * r23_rn;
+    T theta = acos_c(cos_theta);
+    T dtheta = theta - nat_theta;
+    T V = T(0.5) * k * dtheta * dtheta;
+    T dV = k * dtheta;
+    out[0] = -(dV * d1);
+    out[1] = -(dV * d2);
+    out[2] = -(dV * d3);
+    return V;
+}
+// complex_native.cpp:13-49 (kind 0) / simple_native.cpp:13-45 (kind 1)
+template <typename T>
+XMD_HD inline T nat_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, int kind, T CDA, T CDB, T CDH, T nat_phi, v3<T> out[4]) {
+    v3<T> x12_23 = cross(r12, r23), x23_34 = cross(r23, r34);
+    T x12_23_rn = norm_inv(x12_23), x23_34_rn = norm_inv(x23_34);
+    v3<T> x12_23_u = x12_23 * x12_23_rn, x23_34_u = x23_34 * x23_34_rn;
+    T cos_phi = dot(x12_23_u, x23_34_u);
+    T phi = acos_c(cos_phi);
+    if (dot(x12_23, r34) < T(0.0f)) phi = -phi;
+    T diff = phi - nat_phi;
+    T V, dV;
+    if (kind == 0) {
+        T sin_diff = sin(diff), cos_diff = cos(diff);
+        T sin_3diff = sin(T(3.0f) * diff), cos_3diff = cos(T(3.0f) * diff);
+        V = CDA * (T(1.0f) - cos_diff) + CDB * (T(1.0f) - cos_3diff);
+        dV = CDA * sin_diff + CDB * T(3.0f) * sin_3diff;
+    } else {
+        V = T(0.5f) * CDH * diff * diff;
+        dV = CDH * diff;
+    }
+    T r23_n = norm(r23);
+    v3<T> d1 = -x12_23_u * r23_n * x12_23_rn;
+    v3<T> d4 = x23_34_u * r23_n * x23_34_rn;
+    v3<T> df = (-d1 * dot(r12, r23) + d4 * dot(r23, r34)) / (r23_n * r23_n);
+    v3<T> d2 = -d1 + df;
+    v3<T> d3 = -d4 - df;
+    out[0] = -(dV * d1);
+    out[1] = -(dV * d2);
+    out[2] = -(dV * d3);
+    out[3] = -(dV * d4);
+    return V;
+}
+
+// ----------------------------------------------------------------- QA ---
+// n and h of a bead with prev and next (qa/precompute_nh.cpp:12-20); zero
+// otherwise (D17).
+XMD_HD inline void qa_nh(bool ok, v3<double> rp, v3<double> rc, v3<double> rx, v3<double> &n, v3<double> &h) {
+    if (!ok) {
+        n = {0, 0, 0};
+        h = {0, 0, 0};
+        return;
+    }
+    v3<double> v1 = rc - rp, v2 = rx - rc;
+    n = unit(v2 - v1);
+    h = unit(cross(v2, v1));
+}
+struct QaParams {
+    double min_cos_hr, min_cos_hh, max_cos_nr;
+    double cycle_time, cycle_time_inv, factor;
+    double bb_depth, bb_rmin, bs_depth, bs_rmin;
+};
+// sync_data packed as 4 int8 in a uint32 (back, side_all, side_polar,
+// side_hydrophobic), xmd/include/xmd/forces/qa/sync_data.h:8-41.
+XMD_HD inline int8_t sync_get(uint32_t s, int k) { return (int8_t)((s >> (8 * k)) & 0xff); }
+XMD_HD inline uint32_t sync_sub(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; ++k) r |= (uint32_t)(uint8_t)(int8_t)(sync_get(a, k) - sync_get(b, k)) << (8 * k);
+    return r;
+}
+XMD_HD inline uint32_t sync_add(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; ++k) r |= (uint32_t)(uint8_t)(int8_t)(sync_get(a, k) + sync_get(b, k)) << (8 * k);
+    return r;
+}
+XMD_HD inline bool sync_valid(uint32_t s) {
+    return sync_get(s, 0) >= 0 && sync_get(s, 1) >= 0 && sync_get(s, 2) >= 0 && sync_get(s, 3) >= 0;
+}
+XMD_HD inline uint32_t sync_pack(int a, int b, int c, int d) {
+    return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) | ((uint32_t)(uint8_t)d << 24);
+}
+// Candidate test of one free pair, qa/sift_candidates.cpp:15-77.  Returns
+// the contact type (0 = not a candidate) and the two sync diffs.
+XMD_HD inline int qa_sift(const QaParams &Q, const double *req_min_dist, const int8_t *ptype, v3<double> r12,
+                          v3<double> n1, v3<double> h1, v3<double> n2, v3<double> h2, int a1, int a2, uint32_t sync1,
+                          uint32_t sync2, uint32_t &d1, uint32_t &d2) {
+    double r12_rn = norm_inv(r12);
+    v3<double> r12_u = r12 * r12_rn;
+    bool back1 = (fabs(dot(h1, r12_u)) > Q.min_cos_hr);
+    bool back2 = (fabs(dot(h2, r12_u)) > Q.min_cos_hr);
+    bool bb_extra = (fabs(dot(h1, h2)) > Q.min_cos_hh);
+    bool side1 = (dot(n1, r12_u) < Q.max_cos_nr);
+    bool side2 = (-dot(n2, r12_u) < Q.max_cos_nr);
+    int type = 0;
+    if (back1 && back2 && bb_extra) type = 1;
+    else if (back1 && side2) type = 2;
+    else if (side1 && back2) type = 3;
+    else if (side1 && side2) type = 4 + a1 + a2 * 20;
+    if (type == 0) return 0;
+    if (1.0 > r12_rn * req_min_dist[type]) return 0;
+    int pt1 = ptype[a1], pt2 = ptype[a2];
+    d1 = sync_pack(back1, side1, side1 && pt2 == 1, side1 && pt2 == 2);
+    if (!sync_valid(sync_sub(sync1, d1))) return 0;
+    d2 = sync_pack(back2, side2, side2 && pt1 == 1, side2 && pt1 == 2);
+    if (!sync_valid(sync_sub(sync2, d2))) return 0;
+    return type;
+}
+// LJ of a contact type (D5): returns depth, r_min, r_max of the sink_lj.
+XMD_HD inline void qa_lj(const QaParams &Q, const double *ss_depth, const double *ss_rmin, const double *ss_rmax,
+                         int type, double &depth, double &r_min, double &r_max) {
+    if (type == 1) { depth = Q.bb_depth; r_min = r_max = Q.bb_rmin; return; }
+    if (type == 2 || type == 3) { depth = Q.bs_depth; r_min = r_max = Q.bs_rmin; return; }
+    int k = type - 4, a1 = k % 20, a2 = k / 20, idx = 20 * a1 + a2;
+    depth = ss_depth[idx];
+    r_min = ss_rmin[idx];
+    r_max = ss_rmax[idx];
+}
+// Force and transition of one contact, qa/process_contacts.cpp:25-56.
+// Returns the force on i1; *next_status: 0/1 status after the step, 2 =
+// remove.
+XMD_HD inline FV<double> qa_contact(const QaParams &Q, double depth, double r_min, double r_max, v3<double> r12,
+                                    double t, double ref_time, int status, int &next_status) {
+    double r12_rn = norm_inv(r12);
+    v3<double> r12_u = r12 * r12_rn;
+    double tt = t - ref_time;
+    double saturation = (tt < Q.cycle_time ? tt : Q.cycle_time) * Q.cycle_time_inv;
+    if (status == 1) saturation = 1.0f - saturation;
+    double r12_n = r12_rn * norm_sq(r12);
+    VdV<double> e = sink_lj(depth, r_min, r_max, r12_n, r12_rn);
+    FV<double> out;
+    out.V = saturation * e.V;
+    out.F = saturation * e.dV * r12_u;
+    next_status = status;
+    if (status == 0 && saturation == 1.0f) {
+        if (Q.factor * r_min * r12_rn < 1.0f) next_status = 1;
+    } else if (status == 1 && saturation == 0.0f) {
+        next_status = 2;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- RNG ---
+// rand_gen, xmd/include/xmd/random/rand_gen.h:22-35
+XMD_HD inline uint64_t rng_out(uint64_t s) { return s * 0xd989bcacc137dcd5ull; }
+XMD_HD inline uint64_t rng_step(uint64_t s) {
+    s ^= s >> 11;
+    s ^= s << 31;
+    s ^= s >> 18;
+    return s;
+}
+// normal_x2 / normal of rand_gen (rand_gen.h:42-58) with the corrected
+// float libm (bm_math.hpp); fix_table[k] holds the glibc corrections of
+// draw value k (bits 0-1 log, 2-3 sin, 4-5 cos), or nullptr to skip them.
+XMD_HD inline float bm_radius(uint32_t hi, const uint8_t *fix) {
+    float v = (float)hi;
+    float r1 = v * (1.0f / 4294967296.0f);
+    float lg = xmd_bm::logf_cr(r1);
+    if (fix) lg = xmd_bm::apply_fix(lg, fix[xmd_bm::draw_index(v)] & 3u);
+    float arg = -2.0f * lg;
+    float r = (float)sqrt((double)arg);
+    if (r != r) r = 0.0f;
+    return r;
+}
+XMD_HD inline void bm_angle(uint32_t hi, const uint8_t *fix, float &s, float &c) {
+    float v = (float)hi;
+    float r2 = v * (1.0f / 4294967296.0f);
+    float t = (float)(2.0 * M_PI) * r2;
+    double sd, cd;
+    xmd_bm::sincos_red((double)t, sd, cd);
+    s = (float)sd;
+    c = (float)cd;
+    if (fix) {
+        uint32_t code = fix[xmd_bm::draw_index(v)];
+        s = xmd_bm::apply_fix(s, (code >> 2) & 3u);
+        c = xmd_bm::apply_fix(c, (code >> 4) & 3u);
+    }
+}
+// The three noise components of one bead from its first state.
+XMD_HD inline void bead_noise(uint64_t s, const uint8_t *fix, float out[3]) {
+    uint32_t u[4];
+    for (int k = 0; k < 4; ++k) {
+        u[k] = (uint32_t)(rng_out(s) >> 32);
+        s = rng_step(s);
+    }
+    float r = bm_radius(u[0], fix), sn, cs;
+    bm_angle(u[1], fix, sn, cs);
+    out[0] = r * cs;
+    out[1] = r * sn;
+    float r3 = bm_radius(u[2], fix), sn3, cs3;
+    bm_angle(u[3], fix, sn3, cs3);
+    out[2] = r3 * cs3;
+}
+
+}  // namespace xmd_dev

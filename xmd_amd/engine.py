@@ -1,0 +1,81 @@
+"""ctypes driver of libxmdhip.so (the HIP engine, include/xmd_hip.h).
+
+The library must be built (``__graft_entry__.build()`` or
+``python -m xmd_amd.build``); there is no fallback path: a missing library or
+a missing GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._host import HostSystem, XmdError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxmdhip.so")
+_LIB = None
+
+
+def lib():
+    """Load the engine library (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.xmd_last_error.restype = C.c_char_p
+        L.xmd_last_error.argtypes = [C.c_void_p]
+        L.xmd_version.restype = C.c_char_p
+        L.xmd_engine_create.argtypes = [C.POINTER(_abi.XmdParams), C.POINTER(_abi.XmdTopology), C.c_void_p]
+        L.xmd_engine_destroy.argtypes = [C.c_void_p]
+        L.xmd_step.argtypes = [C.c_void_p, C.c_int64]
+        L.xmd_stream.restype = C.c_void_p
+        L.xmd_stream.argtypes = [C.c_void_p]
+        L.xmd_rand_normals.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_uint64)]
+        _LIB = L
+    return _LIB
+
+
+class Engine(HostSystem):
+    """One engine over all replicas of `system` (an xmd_amd.synth.System)."""
+
+    def __init__(self, params, system):
+        super().__init__(params, system)
+        L = lib()
+        h = C.c_void_p()
+        rc = L.xmd_engine_create(C.byref(params), C.byref(self.topo), C.byref(h))
+        if rc != 0:
+            raise XmdError(rc, L.xmd_last_error(None).decode())
+        self.handle = h
+
+    def _fn(self, name):
+        return getattr(lib(), "xmd_" + name)
+
+    def _err(self):
+        return lib().xmd_last_error(self.handle).decode()
+
+    def step(self, nsteps):
+        self._check(lib().xmd_step(self.handle, C.c_int64(nsteps)))
+
+    def stream(self):
+        return lib().xmd_stream(self.handle)
+
+    def rand_normals(self, state, n):
+        out = np.zeros(3 * max(n, 1), dtype=np.float32)
+        so = C.c_uint64()
+        self._check(lib().xmd_rand_normals(self.handle, C.c_uint64(state), C.c_int64(n),
+                                           out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(so)))
+        return out[:3 * n].reshape(n, 3), so.value
+
+    def close(self):
+        if self.handle:
+            lib().xmd_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,327 @@
+"""Seeded synthetic systems for the benchmark configurations C0-C4.
+
+The reference builds its model from seq/PDB files and `morph_into_saw`
+(xmd/src/model/model.cpp:101-174); these configurations instead use chains
+generated here:
+  - native structure: self-avoiding walk, bond 3.8 A, non-bonded beads at
+    least 4.5 A apart; residues uniform over the 20 amino acids;
+  - native contacts: |i-j| >= 3 and d < 7.5 A within a chain, r_min = d_nat;
+  - bonded natives in the conventions of the eval functors: angle
+    theta = acos(-r12.r23/|r12||r23|) (xmd/src/forces/angle/native.cpp:29-30),
+    dihedral as xmd/src/forces/dihedral/complex_native.cpp:19-28,
+    chirality nat_factor = |r23|^-3 and nat_chir = r12.(r23 x r34)*nat_factor
+    (xmd/src/freeform/main.cpp:384-391);
+  - multi-chain boxes: chains placed at random with every inter-chain pair
+    >= 4.5 A under the minimum image.
+Seeds: config c uses 0x5EED0000 + c; replica k of a batch uses that + k.
+Positions are returned in internal units (5 A).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from . import _abi
+from .params import ANGSTROM
+
+BOND = 3.8
+MIN_SEP = 4.5
+NAT_CUT = 7.5
+SEED_BASE = 0x5EED0000
+
+
+def saw_chain(n, rng, bond=BOND, min_sep=MIN_SEP, max_tries=200):
+    """Self-avoiding walk of n beads (A), grown bead by bead with
+    back-tracking when a bead cannot be placed."""
+    pos = np.zeros((n, 3))
+    k = 1
+    fails = np.zeros(n, dtype=np.int64)
+    min2 = min_sep * min_sep
+    while k < n:
+        placed = False
+        for _ in range(max_tries):
+            d = rng.normal(size=3)
+            d *= bond / np.linalg.norm(d)
+            cand = pos[k - 1] + d
+            if k >= 2:
+                diff = pos[:k - 1] - cand
+                if np.min(np.einsum("ij,ij->i", diff, diff)) < min2:
+                    continue
+            pos[k] = cand
+            placed = True
+            break
+        if placed:
+            k += 1
+        else:
+            fails[k] += 1
+            k = max(1, k - 1 - int(fails[k]))
+    return pos
+
+
+def native_contacts(pos, cut=NAT_CUT, min_sep=3):
+    tree = cKDTree(pos)
+    pairs = tree.query_pairs(cut, output_type="ndarray")
+    if len(pairs) == 0:
+        return np.zeros((0, 2), dtype=np.int64), np.zeros(0)
+    pairs = np.sort(pairs, axis=1)
+    pairs = pairs[(pairs[:, 1] - pairs[:, 0]) >= min_sep]
+    order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+    pairs = pairs[order]
+    d = np.linalg.norm(pos[pairs[:, 1]] - pos[pairs[:, 0]], axis=1)
+    return pairs, d
+
+
+def bonded_natives(pos):
+    """Per-bead native angle, dihedral and chirality of one chain (A)."""
+    n = len(pos)
+    theta = np.full(n, np.nan)
+    phi = np.full(n, np.nan)
+    chir = np.full(n, np.nan)
+    chir_f = np.full(n, np.nan)
+    if n >= 3:
+        r12 = pos[1:-1] - pos[:-2]
+        r23 = pos[2:] - pos[1:-1]
+        c = -np.einsum("ij,ij->i", r12, r23) / (np.linalg.norm(r12, axis=1) * np.linalg.norm(r23, axis=1))
+        theta[:n - 2] = np.arccos(np.clip(c, -1, 1))
+    if n >= 4:
+        r12 = pos[1:-2] - pos[:-3]
+        r23 = pos[2:-1] - pos[1:-2]
+        r34 = pos[3:] - pos[2:-1]
+        x12_23 = np.cross(r12, r23)
+        x23_34 = np.cross(r23, r34)
+        u1 = x12_23 / np.linalg.norm(x12_23, axis=1)[:, None]
+        u2 = x23_34 / np.linalg.norm(x23_34, axis=1)[:, None]
+        p = np.arccos(np.clip(np.einsum("ij,ij->i", u1, u2), -1, 1))
+        p = np.where(np.einsum("ij,ij->i", x12_23, r34) < 0, -p, p)
+        phi[:n - 3] = p
+        f = np.linalg.norm(r23, axis=1) ** -3.0
+        chir[:n - 3] = np.einsum("ij,ij->i", r12, x23_34) * f
+        chir_f[:n - 3] = f
+    return theta, phi, chir, chir_f
+
+
+@dataclass
+class System:
+    """Concatenated replicas in the layout of xmd_topology (internal units)."""
+    replica_offset: np.ndarray
+    chain_offset: np.ndarray
+    atype: np.ndarray
+    r0: np.ndarray                 # (3, N)
+    box: np.ndarray                # (R, 3)
+    temperature: np.ndarray        # (R,)
+    rng_state: np.ndarray          # (R,) uint64
+    nat_i1: np.ndarray
+    nat_i2: np.ndarray
+    nat_dist: np.ndarray
+    tether_len: np.ndarray
+    nat_theta: np.ndarray
+    nat_phi: np.ndarray
+    nat_chir: np.ndarray
+    nat_chir_factor: np.ndarray
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_beads(self):
+        return int(self.replica_offset[-1])
+
+    @property
+    def n_replicas(self):
+        return len(self.replica_offset) - 1
+
+    def replica_slice(self, k):
+        return slice(int(self.replica_offset[k]), int(self.replica_offset[k + 1]))
+
+    def topology(self):
+        """XmdTopology pointing into this system's (contiguous) arrays."""
+        arrs = dict(
+            replica_offset=np.ascontiguousarray(self.replica_offset, dtype=np.int32),
+            chain_offset=np.ascontiguousarray(self.chain_offset, dtype=np.int32),
+            atype=np.ascontiguousarray(self.atype, dtype=np.int8),
+            r0=np.ascontiguousarray(self.r0, dtype=np.float64).reshape(-1),
+            box=np.ascontiguousarray(self.box, dtype=np.float64).reshape(-1),
+            temperature=np.ascontiguousarray(self.temperature, dtype=np.float64),
+            rng_state=np.ascontiguousarray(self.rng_state, dtype=np.uint64),
+            nat_i1=np.ascontiguousarray(self.nat_i1, dtype=np.int32),
+            nat_i2=np.ascontiguousarray(self.nat_i2, dtype=np.int32),
+            nat_dist=np.ascontiguousarray(self.nat_dist, dtype=np.float64),
+            tether_len=np.ascontiguousarray(self.tether_len, dtype=np.float64),
+            nat_theta=np.ascontiguousarray(self.nat_theta, dtype=np.float64),
+            nat_phi=np.ascontiguousarray(self.nat_phi, dtype=np.float64),
+            nat_chir=np.ascontiguousarray(self.nat_chir, dtype=np.float64),
+            nat_chir_factor=np.ascontiguousarray(self.nat_chir_factor, dtype=np.float64),
+        )
+        t = _abi.XmdTopology()
+        t.n_replicas = self.n_replicas
+        t.n_beads = self.n_beads
+        t.n_chains = len(self.chain_offset) - 1
+        t.n_native = len(self.nat_i1)
+        types = dict(replica_offset=C.c_int32, chain_offset=C.c_int32, atype=C.c_int8,
+                     rng_state=C.c_uint64, nat_i1=C.c_int32, nat_i2=C.c_int32)
+        for name, a in arrs.items():
+            setattr(t, name, _abi.ptr(a, types.get(name, C.c_double)))
+        t._arrays = arrs  # keep alive with the struct
+        return t
+
+
+def _wrap(x, L):
+    return x - L * np.floor(x / L) if L > 0 else x
+
+
+def place_chains(chains, L, rng, min_sep=MIN_SEP, max_tries=2000):
+    """Translate chains (A) to random positions in a periodic cube of side L
+    with all inter-chain pairs at least min_sep apart."""
+    placed = []
+    tree = None
+    allpos = np.zeros((0, 3))
+    for ch in chains:
+        c0 = ch - ch.mean(axis=0)
+        for _ in range(max_tries):
+            shift = rng.uniform(0, L, size=3)
+            cand = c0 + shift
+            if tree is not None:
+                d, _ = tree.query(_wrap(cand, L), k=1)
+                if np.min(d) < min_sep:
+                    continue
+            break
+        else:
+            raise RuntimeError("could not place chain without overlap")
+        placed.append(cand)
+        allpos = np.vstack([allpos, cand])
+        tree = cKDTree(_wrap(allpos, L), boxsize=L)
+    return placed
+
+
+def make_system(replicas):
+    """replicas: list of dicts with keys chains (list of (pos_A, atype)),
+    box_A (3,), temperature, seed."""
+    rep_off, ch_off = [0], [0]
+    atype, pos, nat_i1, nat_i2, nat_d = [], [], [], [], []
+    theta, phi, chir, chir_f = [], [], [], []
+    box, temp, seeds = [], [], []
+    n = 0
+    for rep in replicas:
+        for p, at in rep["chains"]:
+            m = len(p)
+            atype.append(np.asarray(at, dtype=np.int8))
+            pos.append(p)
+            pairs, d = native_contacts(p)
+            nat_i1.append(pairs[:, 0] + n)
+            nat_i2.append(pairs[:, 1] + n)
+            nat_d.append(d)
+            th, ph, cc, cf = bonded_natives(p)
+            theta.append(th)
+            phi.append(ph)
+            chir.append(cc)
+            chir_f.append(cf)
+            n += m
+            ch_off.append(n)
+        rep_off.append(n)
+        box.append(np.asarray(rep["box_A"], dtype=np.float64))
+        temp.append(rep["temperature"])
+        seeds.append(rep["seed"])
+    pos = np.vstack(pos)
+    return System(
+        replica_offset=np.array(rep_off, dtype=np.int32),
+        chain_offset=np.array(ch_off, dtype=np.int32),
+        atype=np.concatenate(atype),
+        r0=np.ascontiguousarray(pos.T * ANGSTROM),
+        box=np.array(box) * ANGSTROM,
+        temperature=np.array(temp, dtype=np.float64),
+        rng_state=np.array(seeds, dtype=np.uint64),
+        nat_i1=np.concatenate(nat_i1).astype(np.int32),
+        nat_i2=np.concatenate(nat_i2).astype(np.int32),
+        nat_dist=np.concatenate(nat_d) * ANGSTROM,
+        tether_len=np.full(n, np.nan),
+        nat_theta=np.concatenate(theta),
+        nat_phi=np.concatenate(phi),
+        nat_chir=np.concatenate(chir),
+        nat_chir_factor=np.concatenate(chir_f) / ANGSTROM ** 3,
+    )
+
+
+def random_chain(n, rng):
+    return saw_chain(n, rng), rng.integers(0, 20, size=n)
+
+
+# Term sets of the configurations (names of the YAML sections' enabled flags).
+C0_TERMS = dict(go=True, pauli=True, tether=True, chir=True, nat_ang=False, cnd=False, snd=False,
+                const_es=False, rel_es=False, qa=False, pid=False)
+C1_TERMS = dict(C0_TERMS, const_es=True, nat_ang=True, cnd=True)
+FULL_TERMS = dict(go=True, pauli=True, tether=True, chir=True, nat_ang=True, cnd=True, snd=False,
+                  const_es=True, rel_es=False, qa=True, pid=True)
+C3_TERMS = dict(C0_TERMS, const_es=True)
+
+_FLAG = dict(go="go_enabled", pauli="pauli_enabled", tether="tether_enabled", chir="chir_enabled",
+             nat_ang="nat_ang_enabled", cnd="cnd_enabled", snd="snd_enabled", const_es="const_es_enabled",
+             rel_es="rel_es_enabled", qa="qa_enabled", pid="pid_enabled")
+
+
+def apply_terms(p, terms):
+    for k, v in terms.items():
+        setattr(p, _FLAG[k], int(bool(v)))
+    return p
+
+
+def config_params(cid, precision=_abi.PREC_FP64, rng_exact=True):
+    """XmdParams of configuration cid (YAML defaults + the config's terms)."""
+    from .params import from_yaml
+    p, _ = from_yaml(None, precision=precision, rng_exact=rng_exact)
+    p.dt = 0.005
+    p.gamma_factor = 2.0
+    terms = {0: C0_TERMS, 1: C1_TERMS, 2: FULL_TERMS, 3: C3_TERMS, 4: FULL_TERMS}[cid]
+    apply_terms(p, terms)
+    if cid == 0:  # "Verlet cutoff 18 A + 2 A pad"
+        p.max_cutoff_override = 18.0 * ANGSTROM
+        p.pad_factor = 2.0 / 18.0
+    return p
+
+
+def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, seed=None):
+    """Synthetic system of configuration cid.
+
+    C0: 1 x 60, open box.  C1: 1 x 1000, open box.
+    C2: 40 x 250 in a periodic cube at 1e-4 residues/A^3 (L = 464.2 A).
+    C3: 512 single chains, lengths U{30..400}, T ~ U[0.30, 0.50].
+    C4: 400 x 250 at 5e-4 residues/A^3 (L = 584.8 A).
+    n_batch > 1 repeats C2/C4 as independent boxes with temperatures
+    0.30 + 0.2 k/(n_batch-1) and seeds base + k.
+    """
+    base = SEED_BASE + cid if seed is None else seed
+    reps = []
+    if cid in (0, 1):
+        n = {0: 60, 1: 1000}[cid] if chain_len is None else chain_len
+        for k in range(n_batch):
+            rng = np.random.default_rng(base + k)
+            reps.append(dict(chains=[random_chain(n, rng)], box_A=(0, 0, 0), temperature=0.35, seed=base + k))
+    elif cid in (2, 4):
+        nch = {2: 40, 4: 400}[cid] if chains is None else chains
+        m = 250 if chain_len is None else chain_len
+        dens = {2: 1e-4, 4: 5e-4}[cid]
+        L = (nch * m / dens) ** (1.0 / 3.0)
+        for k in range(n_batch):
+            rng = np.random.default_rng(base + k)
+            chs = [random_chain(m, rng) for _ in range(nch)]
+            placed = place_chains([c[0] for c in chs], L, rng)
+            T = 0.35 if n_batch == 1 else 0.30 + 0.2 * k / (n_batch - 1)
+            reps.append(dict(chains=[(p, c[1]) for p, c in zip(placed, chs)], box_A=(L, L, L),
+                             temperature=T, seed=base + k))
+    elif cid == 3:
+        nrep = 512 if n_replicas is None else n_replicas
+        rng0 = np.random.default_rng(base)
+        lens = rng0.integers(30, 401, size=nrep)
+        temps = rng0.uniform(0.30, 0.50, size=nrep)
+        for k in range(nrep):
+            rng = np.random.default_rng(base + 1 + k)
+            reps.append(dict(chains=[random_chain(int(lens[k]), rng)], box_A=(0, 0, 0),
+                             temperature=float(temps[k]), seed=base + 1 + k))
+    else:
+        raise ValueError(cid)
+    return make_system(reps)
+
+
+def perturbed_positions(system, seed, sigma_A=0.3):
+    """Native positions plus isotropic Gaussian displacement (C1's seeded
+    conformations)."""
+    rng = np.random.default_rng(seed)
+    return system.r0 + rng.normal(scale=sigma_A * ANGSTROM, size=system.r0.shape)
