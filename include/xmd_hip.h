@@ -252,6 +252,15 @@ xmd_status xmd_get_counters(xmd_engine *eng, int64_t *out5);
 xmd_status xmd_rand_normals(xmd_engine *eng, uint64_t state, int64_t n,
                             float *out, uint64_t *state_out);
 
+/* Instrumented steps: run nsteps of the loop eagerly (no graph) with HIP
+ * events recorded around every kernel launch on the engine stream.  For
+ * each distinct kernel (at most cap) returns the launch count and the
+ * summed duration in ms; names receives the kernel names, '\n'-separated.
+ * No counterpart in the reference (its only timing is the progress bar,
+ * xmd/src/io/show_progress_bar.cpp:36-51). */
+xmd_status xmd_profile(xmd_engine *eng, int64_t nsteps, char *names, int64_t names_cap,
+                       int64_t *counts, double *total_ms, int32_t cap, int32_t *n_kernels);
+
 /* Stream used by every launch of this engine (hipStream_t as void*). */
 void *xmd_stream(xmd_engine *eng);
 #ifdef __cplusplus

@@ -1,0 +1,371 @@
+"""Pins of the CPU oracle that do not rely on the oracle's own formulas.
+
+The reference has no tests or golden vectors for this path, so each oracle
+function is checked against something the mathematics fixes: closed forms,
+finite-difference gradients, Newton's third law, rotation invariance, an
+independent numpy brute force of the list criterion, glibc called directly
+for the noise transform, and integrator invariants.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from xmd_amd import _abi, synth
+from xmd_amd.params import ANGSTROM
+from oracle import oracle as O
+from oracle.oracle import Oracle
+
+LIBM = C.CDLL("libm.so.6")
+for fn in ("logf", "sinf", "cosf", "sqrtf"):
+    getattr(LIBM, fn).restype = C.c_float
+    getattr(LIBM, fn).argtypes = [C.c_float]
+
+
+# ------------------------------------------------------------------ RNG ---
+MASK = (1 << 64) - 1
+
+
+def xorshift_py(s):
+    out = (s * 0xD989BCACC137DCD5) & MASK
+    s ^= s >> 11
+    s ^= (s << 31) & MASK
+    s ^= s >> 18
+    return out, s
+
+
+def test_rng_stream_matches_closed_form():
+    s = 0x5EED0001
+    so = s
+    for _ in range(1000):
+        v, s = xorshift_py(s)
+        vo, so = O.rng_next(so)
+        assert v == vo and s == so
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def normals_py(state, n):
+    """Box-Muller of rand_gen with glibc float libm called directly."""
+    out = np.zeros((n, 3), dtype=np.float32)
+    s = state
+    two_pi = _f32(2.0 * math.pi)
+    inv = _f32(1.0 / 4294967296.0)
+
+    def uni():
+        nonlocal s
+        v, s = xorshift_py(s)
+        return _f32(_f32(v >> 32) * inv)
+
+    for k in range(n):
+        vals = []
+        for kind in ("x2", "one"):
+            r1, r2 = uni(), uni()
+            lg = LIBM.logf(r1)
+            r = LIBM.sqrtf(_f32(-2.0 * lg))
+            if math.isnan(r):
+                r = 0.0
+            t = _f32(two_pi * r2)
+            vals.append(_f32(r * LIBM.cosf(t)))
+            if kind == "x2":
+                vals.append(_f32(r * LIBM.sinf(t)))
+        out[k] = vals
+    return out, s
+
+
+def test_noise_transform_matches_glibc():
+    for state in (1, 0x5EED0000, 987654321987):
+        a, sa = normals_py(state, 300)
+        b, sb = O.rand_normals(state, 300)
+        assert sa == sb
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_noise_moments():
+    z, _ = O.rand_normals(42, 200000)
+    z = z.astype(np.float64)
+    assert abs(z.mean()) < 0.01
+    assert abs(z.var() - 1.0) < 0.01
+    c = np.corrcoef(z.T)
+    assert np.abs(c - np.eye(3)).max() < 0.01
+
+
+# ----------------------------------------------------------- primitives ---
+def test_lj_closed_forms():
+    V, dV = O.prim(0, 1.5, 2.0, 1 / 2.0)          # at r_min
+    assert V == pytest.approx(-1.5) and abs(dV) < 1e-12
+    V, dV = O.prim(0, 1.0, 2.0, 1 / 4.0)          # r = 2 r_min
+    s6 = 2.0 ** -6
+    assert V == pytest.approx(s6 * s6 - 2 * s6) and dV == pytest.approx(12 / 4.0 * (s6 - s6 * s6))
+    V, dV = O.prim(0, 1.0, 2.0, 1 / 0.5)          # deep core: clamped
+    assert V == 1e3 and dV == -1e3
+
+
+def test_shifted_lj_and_harmonic_zeros():
+    V, dV = O.prim(1, 1.0, 0.8, 1 / 0.8)
+    assert abs(V) < 1e-12 and abs(dV) < 1e-12
+    V, dV = O.prim(3, 100.0, 0.0, 0.76, 0.76)
+    assert V == 0 and dV == 0
+    V, dV = O.prim(3, 100.0, 2.0, 0.76, 0.86)
+    d = 0.1
+    assert V == pytest.approx(0.5 * d * d * (100 + 2 * d * d)) and dV == pytest.approx(d * (100 + 4 * d * d))
+
+
+def test_sink_lj_flat_bottom():
+    for r in (1.2, 1.5, 1.9):
+        V, dV = O.prim(2, 1.0, 1.0, 2.0, r, 1 / r)
+        assert dV == 0.0
+    V, dV = O.prim(2, 1.0, 1.0, 1.0, 1.3, 1 / 1.3)  # r_min == r_max -> plain LJ
+    V2, dV2 = O.prim(0, 1.0, 1.0, 1 / 1.3)
+    assert V == pytest.approx(V2) and dV == pytest.approx(dV2)
+
+
+@pytest.mark.parametrize("kind", [4, 5])
+def test_lambda_value_and_derivative(kind):
+    psi0, alpha = 0.3, 2.5
+    # 16 points, none at psi0 where the algebraic form (a function of |s|) has a cusp
+    for psi in np.linspace(psi0 - 1.2, psi0 + 1.2, 16):
+        v, d = O.prim(kind, psi0, alpha, psi)
+        h = 1e-6
+        vp, _ = O.prim(kind, psi0, alpha, psi + h)
+        vm, _ = O.prim(kind, psi0, alpha, psi - h)
+        assert d == pytest.approx((vp - vm) / (2 * h), rel=1e-6, abs=1e-8)
+    if kind == 4:
+        assert O.prim(4, psi0, alpha, psi0)[0] == pytest.approx(1.0)
+        assert O.prim(4, psi0, alpha, psi0 + math.pi / alpha)[0] == pytest.approx(0.0, abs=1e-12)
+
+
+# -------------------------------------------------- per-term gradients ---
+def _single_term(cid, term, **kw):
+    p = synth.config_params(cid)
+    for name in _abi.TERMS:
+        synth.apply_terms(p, {synth_key(name): False for name in _abi.TERMS if synth_key(name)})
+    synth.apply_terms(p, {synth_key(term): True})
+    return p
+
+
+def synth_key(term):
+    return {"tether": "tether", "chiral": "chir", "nat_ang": "nat_ang", "cnd": "cnd", "snd": "snd",
+            "pauli": "pauli", "go": "go", "const_es": "const_es", "rel_es": "rel_es", "qa": "qa",
+            "pid": "pid"}[term]
+
+
+@pytest.fixture(scope="module")
+def small_open():
+    return synth.config_system(1, chain_len=40)
+
+
+@pytest.fixture(scope="module")
+def small_pbc():
+    return synth.config_system(2, chains=3, chain_len=40)
+
+
+def _compressed(system, factor):
+    """Shrink intra-chain geometry so that repulsive and PID terms are active."""
+    r = system.r0.copy()
+    return r * factor + (1 - factor) * r.mean(axis=1, keepdims=True)
+
+
+FD_TERMS = ["tether", "chiral", "nat_ang", "cnd", "snd", "pauli", "go", "const_es", "rel_es", "pid"]
+
+
+@pytest.mark.parametrize("term", FD_TERMS)
+def test_force_is_minus_gradient(term, small_open, small_pbc):
+    system = small_pbc if term == "pid" else small_open
+    cid = 2 if term == "pid" else 1
+    p = _single_term(cid, term)
+    ora = Oracle(p, system)
+    rng = np.random.default_rng(7)
+    # non-native pairs are >= 7.5 A apart natively: Pauli needs a stronger squeeze
+    r = _compressed(system, 0.5 if term == "pauli" else 0.8) + rng.normal(scale=0.05, size=system.r0.shape)
+    ora.set_positions(0, r)
+    ora.rebuild_lists()
+    F, V = ora.eval_forces()
+    k = _abi.TERM[term]
+    assert np.abs(F).max() > 0, "term inactive in the test geometry"
+    h = 1e-6
+    n = system.n_beads
+    idx = rng.choice(n, size=12, replace=False)
+    for i in idx:
+        for a in range(3):
+            rp = r.copy(); rp[a, i] += h
+            ora.set_positions(0, rp); ora.rebuild_lists()
+            _, Vp = ora.eval_forces()
+            rm = r.copy(); rm[a, i] -= h
+            ora.set_positions(0, rm); ora.rebuild_lists()
+            _, Vm = ora.eval_forces()
+            fd = -(Vp[0, k] - Vm[0, k]) / (2 * h)
+            assert F[a, i] == pytest.approx(fd, rel=1e-5, abs=1e-6 * max(1.0, np.abs(F).max())), (term, i, a)
+
+
+@pytest.mark.parametrize("term", FD_TERMS)
+def test_newton_third_law_and_torque(term, small_open, small_pbc):
+    system = small_pbc if term == "pid" else small_open
+    cid = 2 if term == "pid" else 1
+    ora = Oracle(_single_term(cid, term), system)
+    r = _compressed(system, 0.5 if term == "pauli" else 0.8)
+    ora.set_positions(0, r)
+    F, _ = ora.eval_forces()
+    scale = np.abs(F).max()
+    assert np.abs(F.sum(axis=1)).max() < 1e-9 * max(1, scale)
+    if term != "pid":  # open box: total torque vanishes
+        torque = np.cross(r.T, F.T).sum(axis=0)
+        assert np.abs(torque).max() < 1e-8 * max(1, scale)
+
+
+def test_debye_hueckel_value_at_screening_length():
+    """Two charged beads at r = s: V = Vf q1 q2 e^-1 / s (constant
+    permittivity, decision D9)."""
+    system = synth.make_system([dict(chains=[(np.array([[0, 0, 0], [10.0, 0, 0]]), [1, 3])],
+                                     box_A=(0, 0, 0), temperature=0.3, seed=1)])
+    p = _single_term(1, "const_es")
+    p.max_cutoff_override = 30 * ANGSTROM
+    # chain neighbours are excluded from the list; use two chains instead
+    system = synth.make_system([dict(chains=[(np.array([[0.0, 0, 0]]), [1]), (np.array([[10.0, 0, 0]]), [3])],
+                                     box_A=(0, 0, 0), temperature=0.3, seed=1)])
+    ora = Oracle(p, system)
+    F, V = ora.eval_forces()
+    s = 10 * ANGSTROM
+    Vf = 1 / (4 * math.pi * p.const_es_permittivity)
+    expect = Vf * (1.0 * -1.0) * math.exp(-1.0) / s
+    assert V[0, _abi.TERM["const_es"]] == pytest.approx(expect, rel=1e-12)
+    # force along the axis: -dV/dr = V (1/s + 1/r) pushes... sign check
+    assert F[0, 0] == pytest.approx(-expect * (1 / s + 1 / s), rel=1e-12)
+
+
+# ------------------------------------------------------------- lists ---
+def brute_force_lists(system, p, rep=0):
+    """Independent numpy restatement of the list criterion."""
+    sl = system.replica_slice(rep)
+    r = system.r0[:, sl].T
+    n = r.shape[0]
+    L = system.box[rep]
+    Linv = np.where(L != 0, 1.0 / np.where(L != 0, L, 1), 0.0)
+    chain = np.searchsorted(system.chain_offset, np.arange(sl.start, sl.stop), side="right") - 1
+    # max cutoff as the reference aggregates it (no override in this test)
+    ss = np.array(p.ss_rmax)
+    cands = []
+    if p.pauli_enabled:
+        cands.append(p.pauli_r_excl)
+    if p.go_enabled:
+        cands.append((2.0 * system.nat_dist).max())
+    es_cut = 2.0 * p.const_es_screen
+    if p.const_es_enabled:
+        cands.append(es_cut)
+    mfmd = max(p.lj_bb_rmin, p.lj_bs_rmin, ss.max())
+    if p.qa_enabled:
+        cands.append(mfmd)
+    pid_cut = max(2 * p.lj_bb_rmin, 2 * p.lj_bs_rmin, (2 * ss).max())
+    if p.pid_enabled:
+        cands.append(pid_cut)
+    mc = max(cands)
+    pad = p.pad_factor * mc
+    req = mc + pad
+    native = set(zip(system.nat_i1.tolist(), system.nat_i2.tolist()))
+    out = {k: [] for k in ("nl_all", "nl_native", "pauli", "es", "qa_free")}
+    q = np.array(p.aa_charge)
+    for i in range(n):
+        d = r[i + 1:] - r[i]
+        d = d - L * np.rint(d * Linv)
+        dist = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+        for jj in np.flatnonzero(dist < req):
+            j = i + 1 + jj
+            if chain[i] == chain[j] and j - i < 3:
+                continue
+            out["nl_all"].append((i, j))
+            nat = (i + sl.start, j + sl.start) in native
+            inv = 1.0 / dist[jj]
+            if nat:
+                out["nl_native"].append((i, j))
+            if not nat and p.pauli_enabled and inv > 1.0 / (p.pauli_r_excl + pad):
+                out["pauli"].append((i, j))
+            if p.const_es_enabled and inv > 1.0 / (es_cut + pad) and q[system.atype[i]] * q[system.atype[j]] != 0:
+                out["es"].append((i, j))
+            if not nat and p.qa_enabled and inv > 1.0 / (mfmd + pad):
+                out["qa_free"].append((i, j))
+    return out
+
+
+@pytest.mark.parametrize("cid", [1, 2])
+def test_lists_match_brute_force(cid):
+    system = synth.config_system(1, chain_len=150) if cid == 1 else synth.config_system(2, chains=4, chain_len=60)
+    p = synth.config_params(cid)
+    ora = Oracle(p, system)
+    ora.rebuild_lists()
+    ref = brute_force_lists(system, p)
+    for name, pairs in ref.items():
+        a, b = ora.get_list(0, name)
+        got = list(zip(a.tolist(), b.tolist()))
+        assert got == pairs, name
+
+
+# -------------------------------------------------------- integrator ---
+def _free_beads(n, T=0.35, gamma=2.0):
+    chains = [(np.array([[20.0 * k, 0, 0]]), [0]) for k in range(n)]
+    system = synth.make_system([dict(chains=chains, box_A=(0, 0, 0), temperature=T, seed=11)])
+    p = synth.config_params(0)
+    synth.apply_terms(p, {k: False for k in ("go", "pauli", "tether", "chir", "nat_ang", "cnd", "snd",
+                                             "const_es", "rel_es", "qa", "pid")})
+    p.gamma_factor = gamma
+    return p, system
+
+
+def test_free_flight_is_exact():
+    p, system = _free_beads(4, T=0.0, gamma=0.0)
+    ora = Oracle(p, system)
+    y, _, rng, t = ora.get_state(0)
+    y = y.copy()
+    v = np.array([0.3, -0.2, 0.1])
+    y[1] = (v * p.dt)[:, None] * np.ones((3, 4))
+    ora.set_state(0, y, rng, 0.0)
+    ora.step(100)
+    y2, vel, _, t2 = ora.get_state(0)
+    assert t2 == pytest.approx(100 * p.dt)
+    np.testing.assert_allclose(y2[0], y[0] + 100 * y[1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(vel, v[:, None] * np.ones((3, 4)), rtol=1e-12)
+
+
+def test_friction_decay_matches_exponential():
+    p, system = _free_beads(2, T=0.0, gamma=2.0)
+    ora = Oracle(p, system)
+    y, _, rng, _ = ora.get_state(0)
+    y = y.copy()
+    # scaled derivatives y_k = dt^k/k! x^(k) of v(t) = v0 exp(-gamma t)
+    v0, g, dt = 0.5, 2.0, p.dt
+    for k in range(1, 6):
+        y[k] = dt ** k / math.factorial(k) * v0 * (-g) ** (k - 1)
+    ora.set_state(0, y, rng, 0.0)
+    ora.step(400)
+    _, vel, _, t = ora.get_state(0)
+    assert vel[0, 0] == pytest.approx(v0 * math.exp(-g * t), rel=1e-7)
+
+
+def gear_stationary_velocity_variance(gamma, T, dt):
+    """Exact stationary Var(v) of the Gear-5 recurrence of a free bead driven
+    by the Langevin noise: y <- P (I - c e^T) y + P c (sigma dt^2/2) xi with
+    sigma^2 = 2 gamma T (decision D13: no dt^-1/2), solved as a discrete
+    Lyapunov equation.  P is the Pascal predictor on (y1..y5), c the
+    textbook Gear coefficients, e^T y = y2 + gamma dt/2 y1."""
+    from scipy.linalg import solve_discrete_lyapunov
+    c = np.array([251 / 360, 1.0, 11 / 18, 1 / 6, 1 / 60])
+    e = np.array([gamma * dt / 2, 1, 0, 0, 0])
+    P = np.array([[math.comb(j + 1, i + 1) if j >= i else 0 for j in range(5)] for i in range(5)], dtype=float)
+    A = P @ (np.eye(5) - np.outer(c, e))
+    B = P @ c * (math.sqrt(2 * gamma * T) * dt * dt / 2)
+    S = solve_discrete_lyapunov(A, np.outer(B, B))
+    return S[0, 0] / dt ** 2
+
+
+def test_stationary_velocity_variance():
+    p, system = _free_beads(2000, T=0.4, gamma=2.0)
+    ora = Oracle(p, system)
+    ora.step(1500)
+    acc = []
+    for _ in range(20):
+        ora.step(25)
+        _, vel, _, _ = ora.get_state(0)
+        acc.append((vel ** 2).mean())
+    assert np.mean(acc) == pytest.approx(gear_stationary_velocity_variance(2.0, 0.4, p.dt), rel=0.03)

@@ -162,6 +162,55 @@ struct xmd_engine {
     std::vector<char> pending_rb;
     std::vector<int> nl_base_h, nl_cap_h, fp_base_h, fp_cap_h, ct_base_h, ct_cap_h;
     int nl_total = 0, fp_total = 0, ct_total = 0;
+    // per-kernel event timing (eager profiling pass only)
+    bool prof_on = false;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<int64_t> prof_n;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
+    template <typename L> void launch(const char *name, L fn) {
+        if (!prof_on) {
+            fn();
+            return;
+        }
+        auto ev = [&]() {
+            if (ev_pool.empty()) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                return e;
+            }
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        };
+        hipEvent_t a = ev(), b = ev();
+        HIPCHK(hipEventRecord(a, stream));
+        fn();
+        HIPCHK(hipEventRecord(b, stream));
+        int idx = -1;
+        for (size_t k = 0; k < prof_names.size(); ++k)
+            if (prof_names[k] == name) idx = (int)k;
+        if (idx < 0) {
+            prof_names.push_back(name);
+            prof_ms.push_back(0.0);
+            prof_n.push_back(0);
+            idx = (int)prof_names.size() - 1;
+        }
+        prof_pending.push_back({idx, {a, b}});
+    }
+    void prof_flush() {
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto &pe : prof_pending) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+            prof_ms[pe.first] += ms;
+            prof_n[pe.first] += 1;
+            ev_pool.push_back(pe.second.first);
+            ev_pool.push_back(pe.second.second);
+        }
+        prof_pending.clear();
+    }
 
     template <typename T> T *alloc(size_t n) {
         void *p = nullptr;
@@ -190,6 +239,7 @@ struct xmd_engine {
         return h;
     }
     ~xmd_engine() {
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
         for (auto &g : g_chunk)
             if (g) (void)hipGraphExecDestroy(g);
         for (auto &g : g_one)
@@ -204,30 +254,30 @@ struct xmd_engine {
     // ------------------------------------------------------------ steps ---
     void rebuild_seq(const double *Ppos, int p) {
         int nb = nblk();
-        k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p);
-        k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0);
-        k_cell_fill<<<nb, BLK, 0, stream>>>(D, p);
-        k_nl_count<<<nb, BLK, 0, stream>>>(D, G, Ppos, p);
-        k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 1);
-        k_nl_fill<<<nb, BLK, 0, stream>>>(D, G, Ppos, p);
-        k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2);
-        k_fp_write<<<nb, BLK, 0, stream>>>(D, p);
+        launch("k_cell_count", [&] { k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p); });
+        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0); });
+        launch("k_cell_fill", [&] { k_cell_fill<<<nb, BLK, 0, stream>>>(D, p); });
+        launch("k_nl_count", [&] { k_nl_count<<<nb, BLK, 0, stream>>>(D, G, Ppos, p); });
+        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 1); });
+        launch("k_nl_fill", [&] { k_nl_fill<<<nb, BLK, 0, stream>>>(D, G, Ppos, p); });
+        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2); });
+        launch("k_fp_write", [&] { k_fp_write<<<nb, BLK, 0, stream>>>(D, p); });
     }
     template <typename T> void force_launch(const TermTab<T> &TT, int p, bool fused) {
         int nb = nblk();
-        if (fused) k_force<T, true, false><<<nb, BLK, 0, stream>>>(D, G, TT, p);
-        else k_force<T, false, false><<<nb, BLK, 0, stream>>>(D, G, TT, p);
+        if (fused) launch("k_force_fused", [&] { k_force<T, true, false><<<nb, BLK, 0, stream>>>(D, G, TT, p); });
+        else launch("k_force", [&] { k_force<T, false, false><<<nb, BLK, 0, stream>>>(D, G, TT, p); });
     }
     void step_seq(int p) {
         if (G.qa_on) {
-            k_qa_pairs<<<qa_grid, BLK, 0, stream>>>(D, G, p, nfp_chunks, nct_chunks, 1);
-            k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p);
+            launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, 0, stream>>>(D, G, p, nfp_chunks, nct_chunks, 1); });
+            launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
         }
         bool fused = !pid_split;
         if (P.precision == XMD_PREC_MIXED) force_launch(TTf, p, fused);
         else force_launch(TTd, p, fused);
-        if (!fused) k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p);
-        k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1);
+        if (!fused) launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
+        launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1); });
         rebuild_seq(D.pos[1 - p], p);
     }
     hipGraphExec_t capture(int p, int nsteps) {
@@ -1079,6 +1129,48 @@ xmd_status xmd_get_counters(xmd_engine *e, int64_t *out) {
         out[2] = nl;
         out[3] = e->nl_total;
         out[4] = qa;
+    });
+}
+
+xmd_status xmd_profile(xmd_engine *e, int64_t nsteps, char *names, int64_t names_cap, int64_t *counts,
+                       double *total_ms, int32_t cap, int32_t *n_kernels) {
+    if (!e || nsteps < 0 || !n_kernels) return XMD_ERR_ARG;
+    return run_guarded(e->err, [&]() {
+        e->ensure_lists();
+        e->prof_names.clear();
+        e->prof_ms.clear();
+        e->prof_n.clear();
+        e->prof_on = true;
+        try {
+            for (int64_t k = 0; k < nsteps; ++k) {
+                e->step_seq(e->cur);
+                e->cur ^= 1;
+                e->steps += 1;
+                if ((k & 63) == 63) e->prof_flush();
+            }
+            e->prof_flush();
+        } catch (...) {
+            e->prof_on = false;
+            throw;
+        }
+        e->prof_on = false;
+        HIPCHK(hipGetLastError());
+        e->check_err();
+        int nk = (int)e->prof_names.size();
+        *n_kernels = nk;
+        std::string all;
+        for (int k = 0; k < nk; ++k) {
+            if (k < cap) {
+                if (counts) counts[k] = e->prof_n[k];
+                if (total_ms) total_ms[k] = e->prof_ms[k];
+            }
+            all += e->prof_names[k] + "\n";
+        }
+        if (names && names_cap > 0) {
+            size_t m = std::min<size_t>(all.size(), (size_t)names_cap - 1);
+            std::memcpy(names, all.data(), m);
+            names[m] = 0;
+        }
     });
 }
 

@@ -79,3 +79,18 @@ class Engine(HostSystem):
             self.close()
         except Exception:
             pass
+
+
+def profile(engine, nsteps):
+    """Per-kernel HIP-event timing of nsteps eager steps:
+    {name: (launches, total_ms)}."""
+    cap = 64
+    names = C.create_string_buffer(8192)
+    counts = np.zeros(cap, dtype=np.int64)
+    ms = np.zeros(cap)
+    nk = C.c_int32()
+    engine._check(lib().xmd_profile(engine.handle, C.c_int64(nsteps), names, C.c_int64(8192),
+                                    counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    ms.ctypes.data_as(C.POINTER(C.c_double)), C.c_int32(cap), C.byref(nk)))
+    lst = names.value.decode().split("\n")
+    return {lst[k]: (int(counts[k]), float(ms[k])) for k in range(nk.value)}
