@@ -232,30 +232,68 @@ __device__ inline void slot_remove(const Dev &D, int s, int ci) {
     D.qn[s] = q - 1;
 }
 
+// Collect, in order, the chunks [c_lo, c_hi) whose counter is non-zero into
+// s_list (at most BLK per call, starting at chunk `from`); returns how many
+// were listed and sets *next to the first chunk not examined.
+__device__ inline int list_nonzero_chunks(const int *cnt, int c_lo, int c_hi, int from, int *s_list, int *s_n,
+                                          int *next) {
+    __shared__ int s_w[BLK / WAVE];
+    int tid = threadIdx.x;
+    int c = from + tid;
+    bool nz = c < c_hi && cnt[c] > 0;
+    unsigned long long m = __ballot(nz);
+    int w = tid / WAVE;
+    if (lane_id() == 0) s_w[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int k = 0; k < BLK / WAVE; ++k) {
+        if (k < w) off += s_w[k];
+        tot += s_w[k];
+    }
+    if (nz) s_list[off + __popcll(m & ((1ull << lane_id()) - 1ull))] = c;
+    if (tid == 0) *s_n = tot;
+    __syncthreads();
+    *next = from + BLK;
+    (void)c_lo;
+    return *s_n;
+}
+
 // One workgroup per replica: process candidates in free-pair order, then
 // the removals of process_contacts in contact order, then compact the
-// contact set when its extent passes half the capacity.
+// contact set when its extent passes half the capacity.  The workgroup
+// lists the non-empty chunks; one thread applies their entries serially.
 __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
     __shared__ int s_scan[BLK];
+    __shared__ int s_list[BLK];
+    __shared__ int s_n;
+    __shared__ int s_state[4];  // ct count, fp count, cnt0
     int rep = blockIdx.x, tid = threadIdx.x;
     int fpb = D.fp_base[rep], ctb = D.ct_base[rep];
-    int nfpc = (D.fp_count[rep] + BLK - 1) / BLK;
+    int fc_lo = fpb / BLK, fc_hi = fc_lo + (D.fp_count[rep] + BLK - 1) / BLK;
     double t = D.tt[p][rep];
-    // candidates
     if (tid == 0) {
-        int cnt = D.ct_count[rep];
-        int cnt0 = cnt;
-        int cap = D.ct_cap[rep];
-        for (int c = 0; c < nfpc; ++c) {
-            int chunk = fpb / BLK + c;
-            int nc = D.cand_cnt[chunk];
-            for (int k = 0; k < nc; ++k) {
-                int e = chunk * BLK + k;
-                int fp = D.cand_fp[e];
-                int i1 = D.fp_i1[fp], i2 = D.fp_i2[fp];
-                uint32_t d1 = D.cand_d1[e], d2 = D.cand_d2[e];
-                uint32_t ns1 = sync_sub(D.sync[i1], d1), ns2 = sync_sub(D.sync[i2], d2);
-                if (sync_valid(ns1) && sync_valid(ns2)) {
+        s_state[0] = D.ct_count[rep];
+        s_state[1] = D.fp_count[rep];
+        s_state[2] = s_state[0];
+    }
+    __syncthreads();
+    const int cap = D.ct_cap[rep], tot = D.ct_cap[D.nrep], fcap = D.fp_cap[rep];
+    // candidates, in free-pair order
+    for (int from = fc_lo; from < fc_hi;) {
+        int nxt;
+        int nl = list_nonzero_chunks(D.cand_cnt, fc_lo, fc_hi, from, s_list, &s_n, &nxt);
+        if (tid == 0) {
+            int cnt = s_state[0];
+            for (int q = 0; q < nl; ++q) {
+                int chunk = s_list[q];
+                int nc = D.cand_cnt[chunk];
+                for (int k = 0; k < nc; ++k) {
+                    int e = chunk * BLK + k;
+                    int fp = D.cand_fp[e];
+                    int i1 = D.fp_i1[fp], i2 = D.fp_i2[fp];
+                    uint32_t d1 = D.cand_d1[e], d2 = D.cand_d2[e];
+                    uint32_t ns1 = sync_sub(D.sync[i1], d1), ns2 = sync_sub(D.sync[i2], d2);
+                    if (!(sync_valid(ns1) && sync_valid(ns2))) continue;
                     if (cnt >= cap) {
                         atomicOr(D.err, E_CT_CAP);
                         break;
@@ -269,57 +307,67 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
                     D.ct_d1[ci] = d1;
                     D.ct_d2[ci] = d2;
                     D.ct_vac[ci] = 0;
-                    int tot = D.ct_cap[D.nrep];
                     D.ct_F[ci] = 0;
                     D.ct_F[tot + ci] = 0;
                     D.ct_F[2 * tot + ci] = 0;
                     D.ct_V[ci] = 0;
                     D.sync[i1] = ns1;
-                    D.sync[i2] = sync_sub(D.sync[i2], d2);
+                    D.sync[i2] = ns2;
                     slot_add(D, i1, ci);
                     slot_add(D, i2, ci);
                     D.fp_vac[fp] = 1;
                 }
             }
-            D.cand_cnt[chunk] = 0;
+            s_state[0] = cnt;
         }
-        // removals, in contact order (contacts formed above have no removal)
-        int nctc = (cnt0 + BLK - 1) / BLK;
-        int fcnt = D.fp_count[rep];
-        int fcap = D.fp_cap[rep];
-        for (int c = 0; c < nctc; ++c) {
-            int chunk = ctb / BLK + c;
-            int nr = D.rm_cnt[chunk];
-            for (int k = 0; k < nr; ++k) {
-                int ci = D.rm_idx[chunk * BLK + k];
-                int i1 = D.ct_i1[ci], i2 = D.ct_i2[ci];
-                D.ct_vac[ci] = 1;
-                D.sync[i1] = sync_add(D.sync[i1], D.ct_d1[ci]);
-                D.sync[i2] = sync_add(D.sync[i2], D.ct_d2[ci]);
-                if (fcnt >= fcap) {
-                    atomicOr(D.err, E_FP_CAP);
-                } else {
-                    int f = fpb + fcnt++;
-                    D.fp_i1[f] = i1;
-                    D.fp_i2[f] = i2;
-                    D.fp_vac[f] = 0;
+        __syncthreads();
+        from = nxt;
+    }
+    // removals, in contact order (contacts formed above have none)
+    int cc_lo = ctb / BLK, cc_hi = cc_lo + (s_state[2] + BLK - 1) / BLK;
+    for (int from = cc_lo; from < cc_hi;) {
+        int nxt;
+        int nl = list_nonzero_chunks(D.rm_cnt, cc_lo, cc_hi, from, s_list, &s_n, &nxt);
+        if (tid == 0) {
+            int fcnt = s_state[1];
+            for (int q = 0; q < nl; ++q) {
+                int chunk = s_list[q];
+                int nr = D.rm_cnt[chunk];
+                for (int k = 0; k < nr; ++k) {
+                    int ci = D.rm_idx[chunk * BLK + k];
+                    int i1 = D.ct_i1[ci], i2 = D.ct_i2[ci];
+                    D.ct_vac[ci] = 1;
+                    D.sync[i1] = sync_add(D.sync[i1], D.ct_d1[ci]);
+                    D.sync[i2] = sync_add(D.sync[i2], D.ct_d2[ci]);
+                    if (fcnt >= fcap) {
+                        atomicOr(D.err, E_FP_CAP);
+                    } else {
+                        int f = fpb + fcnt++;
+                        D.fp_i1[f] = i1;
+                        D.fp_i2[f] = i2;
+                        D.fp_vac[f] = 0;
+                    }
+                    slot_remove(D, i1, ci);
+                    slot_remove(D, i2, ci);
                 }
-                slot_remove(D, i1, ci);
-                slot_remove(D, i2, ci);
+                D.rm_cnt[chunk] = 0;
             }
-            D.rm_cnt[chunk] = 0;
+            s_state[1] = fcnt;
         }
-        D.fp_count[rep] = fcnt;
-        D.ct_count[rep] = cnt;
-        s_scan[0] = cnt > D.ct_cap[rep] / 2;
+        __syncthreads();
+        from = nxt;
+    }
+    if (tid == 0) {
+        D.fp_count[rep] = s_state[1];
+        D.ct_count[rep] = s_state[0];
+        s_scan[0] = s_state[0] > cap / 2;
     }
     __syncthreads();
     if (!s_scan[0]) return;
     __syncthreads();
     // compaction: in-place, chunk by chunk in index order (new <= old)
-    int cnt = D.ct_count[rep];
+    int cnt = s_state[0];
     int kept = 0;
-    int tot = D.ct_cap[D.nrep];
     for (int c0 = 0; c0 < cnt; c0 += BLK) {
         int o = c0 + tid;
         bool keep = o < cnt && !D.ct_vac[ctb + o];
