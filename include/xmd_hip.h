@@ -197,6 +197,13 @@ void xmd_engine_destroy(xmd_engine *eng);
 const char *xmd_last_error(const xmd_engine *eng);
 /* Library version string. */
 const char *xmd_version(void);
+/* Index of the first of n_draws rand_gen draws from `state` whose top 32
+ * bits are zero and whose index mod 4 is set in slot_mask, or -1.
+ * Draw 4*(N*k + i) + {0, 2} (slot_mask 5) of a replica of N beads
+ * feeds log() for bead i at step k; a zero there makes the reference's
+ * noise infinite (rand_gen.h:45-46), which the engine reproduces and
+ * reports as XMD_ERR_NONFINITE.  Host-only. */
+int64_t xmd_rng_first_zero_draw(uint64_t state, int64_t n_draws, uint32_t slot_mask);
 
 /* Dynamic state of one replica: Gear derivatives y0..y5 (lang_pc.h:16),
  * y = [y0x,y0y,y0z,y1x,...,y5z] each block of n beads; rand_gen state;

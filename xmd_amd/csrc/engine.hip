@@ -257,26 +257,24 @@ struct xmd_engine {
         launch("k_cell_count", [&] { k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0); });
         launch("k_cell_fill", [&] { k_cell_fill<<<nb, BLK, 0, stream>>>(D, p); });
-        launch("k_nl_count", [&] { k_nl_count<<<nb, BLK, 0, stream>>>(D, G, Ppos, p); });
+        launch("k_nl_count", [&] { k_nl_count<<<nslot / NL_WAVES, BLK, 0, stream>>>(D, G, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 1); });
-        launch("k_nl_fill", [&] { k_nl_fill<<<nb, BLK, 0, stream>>>(D, G, Ppos, p); });
+        launch("k_nl_fill", [&] { k_nl_fill<<<nslot / NL_WAVES, BLK, 0, stream>>>(D, G, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2); });
         launch("k_fp_write", [&] { k_fp_write<<<nb, BLK, 0, stream>>>(D, p); });
     }
-    template <typename T> void force_launch(const TermTab<T> &TT, int p, bool fused) {
-        int nb = nblk();
-        if (fused) launch("k_force_fused", [&] { k_force<T, true, false><<<nb, BLK, 0, stream>>>(D, G, TT, p); });
-        else launch("k_force", [&] { k_force<T, false, false><<<nb, BLK, 0, stream>>>(D, G, TT, p); });
+    static constexpr int LPB = 8;  // lanes per bead in k_force
+    template <typename T> void force_launch(const TermTab<T> &TT, int p) {
+        launch("k_force", [&] { k_force<T, false, LPB><<<nslot * LPB / BLK, BLK, 0, stream>>>(D, G, TT, p); });
     }
     void step_seq(int p) {
         if (G.qa_on) {
             launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, 0, stream>>>(D, G, p, nfp_chunks, nct_chunks, 1); });
             launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
         }
-        bool fused = !pid_split;
-        if (P.precision == XMD_PREC_MIXED) force_launch(TTf, p, fused);
-        else force_launch(TTd, p, fused);
-        if (!fused) launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
+        if (P.precision == XMD_PREC_MIXED) force_launch(TTf, p);
+        else force_launch(TTd, p);
+        launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
         launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1); });
         rebuild_seq(D.pos[1 - p], p);
     }
@@ -310,6 +308,9 @@ struct xmd_engine {
             if (e[0] & E_FP_CAP) m += " QA free-pair capacity exceeded;";
             if (e[0] & E_CT_CAP) m += " QA contact capacity exceeded;";
             if (e[0] & E_SLOT) m += " more QA contacts on a bead than slots;";
+            if (e[0] & E_ZERO_DRAW)
+                throw Fail{XMD_ERR_NONFINITE, "rand_gen produced a zero uniform draw for log(): infinite Langevin "
+                                              "noise, as in the reference (rand_gen.h:45-46); positions are not finite"};
             if (e[0] & E_NONFINITE) throw Fail{XMD_ERR_NONFINITE, "non-finite positions"};
             throw Fail{XMD_ERR_CAPACITY, m};
         }
@@ -604,6 +605,8 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
         rows_of(J256, jrow.data() + 64 * j);
         J256 = mul(J256, J256);
     }
+    std::vector<uint64_t> jadv(64 * (size_t)E.nrep);
+    for (int k = 0; k < E.nrep; ++k) rows_of(mpow(J, 4ull * (uint64_t)E.rep_n[k]), jadv.data() + 64 * (size_t)k);
     Mat J4 = mpow(J, 4), Jl = identity();
     for (int l = 0; l < 64; ++l) {
         for (int b = 0; b < 64; ++b) jlane[64 * b + l] = Jl.col[b];
@@ -672,6 +675,7 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     D.ptype = E.upload(ptype);
     D.jrow = E.upload(jrow);
     D.jlane = E.upload(jlane);
+    D.jadv = E.upload(jadv);
     if (P.rng_exact) D.bm_fix = E.upload(fix_table());
     int nsl = n / WAVE;
     D.slice_start = E.alloc<int>(nsl);
@@ -728,9 +732,11 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     k_cell_count<<<nb, BLK, 0, E.stream>>>(D, D.pos[0], 0);
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 0);
     k_cell_fill<<<nb, BLK, 0, E.stream>>>(D, 0);
-    k_nl_count<<<nb, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
+    k_nl_count<<<n / NL_WAVES, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
     HIPCHK(hipGetLastError());
-    auto w = E.get(D.slice_w, nsl);
+    auto len = E.get(D.nl_len, n);
+    std::vector<int> w(nsl, 0);
+    for (int s = 0; s < n; ++s) w[s / WAVE] = std::max(w[s / WAVE], len[s]);
     E.nl_base_h.resize(E.nrep + 1);
     E.nl_cap_h.resize(E.nrep + 1);
     E.fp_base_h.resize(E.nrep + 1);
@@ -778,7 +784,7 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     E.qa_grid = std::max(1, std::min(E.nfp_chunks + E.nct_chunks, 8192));
     // finish the first rebuild
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 1);
-    k_nl_fill<<<nb, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
+    k_nl_fill<<<n / NL_WAVES, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 2);
     k_fp_write<<<nb, BLK, 0, E.stream>>>(D, 0);
     HIPCHK(hipGetLastError());
@@ -803,6 +809,14 @@ xmd_status run_guarded(std::string &err, const std::function<void()> &fn) {
 extern "C" {
 
 const char *xmd_version(void) { return "xmd-hip 0.1.0 gfx950"; }
+
+int64_t xmd_rng_first_zero_draw(uint64_t state, int64_t n_draws, uint32_t slot_mask) {
+    for (int64_t k = 0; k < n_draws; ++k) {
+        if (((slot_mask >> (k & 3)) & 1u) && (xmd_dev::rng_out(state) >> 32) == 0) return k;
+        state = xmd_dev::rng_step(state);
+    }
+    return -1;
+}
 
 const char *xmd_last_error(const xmd_engine *e) { return e ? e->err.c_str() : g_create_err.c_str(); }
 
@@ -967,11 +981,11 @@ xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double 
         if (e->P.precision == XMD_PREC_MIXED) {
             TermTab<float> TT = e->TTf;
             TT.mask = m;
-            k_force<float, false, true><<<nb, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
+            k_force<float, true, xmd_engine::LPB><<<e->nslot * xmd_engine::LPB / BLK, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
         } else {
             TermTab<double> TT = e->TTd;
             TT.mask = m;
-            k_force<double, false, true><<<nb, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
+            k_force<double, true, xmd_engine::LPB><<<e->nslot * xmd_engine::LPB / BLK, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
         }
         k_gather_force<<<nb, BLK, 0, e->stream>>>(e->D);
         HIPCHK(hipGetLastError());

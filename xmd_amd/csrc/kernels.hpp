@@ -29,7 +29,7 @@ constexpr int NUM_TERMS = 11;
 
 enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16 };
 enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
-enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16 };
+enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
 enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID };
 
 // Derived parameters in double (list and QA decisions, integrator).
@@ -92,7 +92,7 @@ struct Dev {
     const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
     const int8_t *ptype;
     const uint8_t *bm_fix;
-    const uint64_t *jrow, *jlane;
+    const uint64_t *jrow, *jlane, *jadv;  // jadv: rows of J^(4N) per replica
 };
 
 __device__ inline v3<double> ld3(const double *a, int n, int s) { return {a[s], a[n + s], a[2 * n + s]}; }
@@ -425,6 +425,8 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
     uint64_t S = wave_lane_state(D, D.rng[p][rep], local >> 6);
     float nz[3];
     bead_noise(S, D.bm_fix, nz);
+    // a zero uniform in a log() slot gives infinite noise (rand_gen.h:45-46)
+    if (active && (!((rng_out(S) >> 32)) || !((rng_out(rng_step(rng_step(S))) >> 32)))) atomicOr(D.err, E_ZERO_DRAW);
     double disp = 0.0;
     if (active) {
         double *y = D.y;
@@ -471,7 +473,12 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
 }
 
 // ------------------------------------------------------------- forces ---
-// Force on slot s from every enabled term; FUSED then integrates s.
+// Force on every slot from every enabled term.  LPB consecutive lanes share
+// a bead: the 8 bonded tasks (4 quads, 3 angles, the two tethers) and the
+// neighbour-row entries are strided over the lanes, then a shuffle tree
+// (fixed order, hence deterministic) sums the lanes' partial forces.  The
+// result goes to Fbuf (plus the PID spill onto chain neighbours); the
+// integrator runs in k_integrate.
 template <typename T> struct TermTab {
     TermParams<T> P;
     uint32_t mask;
@@ -484,16 +491,22 @@ __device__ inline void add3(double &x, double &y, double &z, v3<T> f) {
     z += (double)f.z;
 }
 
-template <typename T, bool FUSED, bool EVAL>
+template <int LPB> __device__ inline double lane_sum(double v) {
+    for (int o = LPB / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <typename T, bool EVAL, int LPB>
 __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
     const int n = D.nslot;
-    int s = blockIdx.x * BLK + threadIdx.x;
-    int rep = D.slot_rep[s];
-    uint8_t bf = D.bflags[s];
-    bool active = bf & B_ACTIVE;
+    const int t = blockIdx.x * BLK + threadIdx.x;
+    const int s = t / LPB, sub = t % LPB;
+    const int rep = D.slot_rep[s];
+    const uint8_t bf = D.bflags[s];
+    const bool active = bf & B_ACTIVE;
     const double *Pp = D.pos[p];
     const TermParams<T> &P = TT.P;
-    uint32_t mask = TT.mask;
+    const uint32_t mask = TT.mask;
     double Fx = 0, Fy = 0, Fz = 0, V = 0;
     double Vt[EVAL ? NUM_TERMS : 1];
     for (int k = 0; k < (EVAL ? NUM_TERMS : 1); ++k) Vt[k] = 0;
@@ -502,55 +515,21 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
         if constexpr (EVAL) Vt[term] += v;
     };
     double tpx = 0, tpy = 0, tpz = 0, tnx = 0, tny = 0, tnz = 0;  // PID spill to s-1, s+1
-    v3<double> ri = {0, 0, 0};
     if (active) {
-        ri = ld3(Pp, n, s);
-        Box box = D.box[rep];
-        int ch = D.chain[s];
-        // chain window s-3 .. s+3
-        v3<double> rw[7];
-        bool inw[7];
-        for (int k = 0; k < 7; ++k) {
-            int a = s + k - 3;
-            inw[k] = a >= 0 && a < n && D.chain[a] == ch;
-            rw[k] = inw[k] ? ld3(Pp, n, a) : ri;
-        }
-        // tether (i, i+1)
-        if (mask & (1u << T_TETHER)) {
-            v3<T> out[2];
-            if (inw[2]) {
-                double nat = D.tether_nat[s - 1];
-                tether<T>(cvt<T>(rw[3] - rw[2]), P.H1, P.H2, (T)nat, out);
-                add3(Fx, Fy, Fz, out[1]);
-            }
-            if (inw[4]) {
-                double nat = D.tether_nat[s];
-                T v = tether<T>(cvt<T>(rw[4] - rw[3]), P.H1, P.H2, (T)nat, out);
-                add3(Fx, Fy, Fz, out[0]);
-                addV(T_TETHER, (double)v);
-            }
-        }
-        // angles (a, a+1, a+2), a = s-2 .. s
-        if (mask & (1u << T_NAT_ANG)) {
-            for (int a = 0; a < 3; ++a) {
-                int k0 = a + 1;  // window index of the first bead (s-2 -> 1)
-                if (!inw[k0] || !inw[k0 + 2]) continue;
-                double th = D.theta_nat[s - 2 + a];
-                if (th != th) continue;
-                v3<T> out[3];
-                T v = nat_angle<T>(cvt<T>(rw[k0 + 1] - rw[k0]), cvt<T>(rw[k0 + 2] - rw[k0 + 1]), P.ang_k, (T)th, out);
-                add3(Fx, Fy, Fz, out[2 - a]);
-                if (a == 2) addV(T_NAT_ANG, (double)v);
-            }
-        }
-        // quads (a .. a+3), a = s-3 .. s: chirality and native dihedrals
-        uint32_t qmask = mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND));
-        if (qmask) {
-            for (int a = 0; a < 4; ++a) {
-                if (!inw[a] || !inw[a + 3]) continue;
-                int first = s - 3 + a;
-                v3<T> r12 = cvt<T>(rw[a + 1] - rw[a]), r23 = cvt<T>(rw[a + 2] - rw[a + 1]),
-                      r34 = cvt<T>(rw[a + 3] - rw[a + 2]);
+        const v3<double> ri = ld3(Pp, n, s);
+        const Box box = D.box[rep];
+        const int ch = D.chain[s];
+        auto in_chain = [&](int a) { return a >= 0 && a < n && D.chain[a] == ch; };
+        // bonded tasks: 0..3 quad starting at s-3+task, 4..6 angle starting
+        // at s-2+(task-4), 7 tethers
+        for (int task = sub; task < 8; task += LPB) {
+            if (task < 4) {
+                int a = task, first = s - 3 + a;
+                if (!(mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND)))) continue;
+                if (!in_chain(first) || !in_chain(first + 3)) continue;
+                v3<double> q0 = ld3(Pp, n, first), q1 = ld3(Pp, n, first + 1), q2 = ld3(Pp, n, first + 2),
+                           q3 = ld3(Pp, n, first + 3);
+                v3<T> r12 = cvt<T>(q1 - q0), r23 = cvt<T>(q2 - q1), r34 = cvt<T>(q3 - q2);
                 v3<T> out[4];
                 if (mask & (1u << T_CHIRAL)) {
                     double nc = D.chir_nat[first];
@@ -573,11 +552,34 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
                         if (a == 3) addV(T_SND, (double)v);
                     }
                 }
+            } else if (task < 7) {
+                int a = task - 4, first = s - 2 + a;
+                if (!(mask & (1u << T_NAT_ANG))) continue;
+                if (!in_chain(first) || !in_chain(first + 2)) continue;
+                double th = D.theta_nat[first];
+                if (th != th) continue;
+                v3<double> q0 = ld3(Pp, n, first), q1 = ld3(Pp, n, first + 1), q2 = ld3(Pp, n, first + 2);
+                v3<T> out[3];
+                T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
+                add3(Fx, Fy, Fz, out[2 - a]);
+                if (a == 2) addV(T_NAT_ANG, (double)v);
+            } else {
+                if (!(mask & (1u << T_TETHER))) continue;
+                v3<T> out[2];
+                if (bf & B_PREV) {
+                    tether<T>(cvt<T>(ri - ld3(Pp, n, s - 1)), P.H1, P.H2, (T)D.tether_nat[s - 1], out);
+                    add3(Fx, Fy, Fz, out[1]);
+                }
+                if (bf & B_NEXT) {
+                    T v = tether<T>(cvt<T>(ld3(Pp, n, s + 1) - ri), P.H1, P.H2, (T)D.tether_nat[s], out);
+                    add3(Fx, Fy, Fz, out[0]);
+                    addV(T_TETHER, (double)v);
+                }
             }
         }
         // native contacts (Go)
         if (mask & (1u << T_GO)) {
-            for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k) {
+            for (int k = D.nat_start[s] + sub; k < D.nat_start[s + 1]; k += LPB) {
                 if (!D.nat_act[k]) continue;
                 int j = D.nat_j[k];
                 v3<double> d = wrap(box, ld3(Pp, n, j) - ri);
@@ -589,20 +591,22 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
         // neighbour row: Pauli, Debye-Hueckel, PID
         uint32_t rmask = mask & ((1u << T_PAULI) | (1u << T_CONST_ES) | (1u << T_REL_ES) | (1u << T_PID));
         if (rmask && !(D.err[0] & E_NL_CAP)) {
-            int len = D.nl_len[s];
-            int base = D.slice_start[s >> 6] + (s & 63);
-            int es_term = P.es_relative ? T_REL_ES : T_CONST_ES;
-            bool es_on = mask & (1u << es_term);
-            T q_s = (T)D.charge[D.atype[s]];
-            for (int k = 0; k < len; ++k) {
-                int e = base + 64 * k;
-                uint8_t fl = D.nl_flag[e];
+            const int len = D.nl_len[s];
+            const int base = D.slice_start[s >> 6] + (s & 63);
+            const int es_term = P.es_relative ? T_REL_ES : T_CONST_ES;
+            const bool es_on = mask & (1u << es_term);
+            const T q_s = (T)D.charge[D.atype[s]];
+            const v3<double> rsp = (bf & B_PREV) ? ld3(Pp, n, s - 1) : ri;
+            const v3<double> rsn = (bf & B_NEXT) ? ld3(Pp, n, s + 1) : ri;
+            for (int k = sub; k < len; k += LPB) {
+                const int e = base + 64 * k;
+                const uint8_t fl = D.nl_flag[e];
                 if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
-                int j = D.nl_j[e];
-                v3<double> rj = ld3(Pp, n, j);
-                v3<double> dd = wrap(box, rj - ri);
-                v3<T> d = cvt<T>(dd);
-                bool owner = s < j;
+                const int j = D.nl_j[e];
+                const v3<double> rj = ld3(Pp, n, j);
+                const v3<double> dd = wrap(box, rj - ri);
+                const v3<T> d = cvt<T>(dd);
+                const bool owner = s < j;
                 if ((fl & F_PAULI) && (mask & (1u << T_PAULI))) {
                     FV<T> f = pauli_pair<T>(d, P.pauli_depth, P.pauli_r_excl);
                     add3(Fx, Fy, Fz, f.F);
@@ -615,17 +619,18 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
                     if (owner) addV(es_term, (double)f.V);
                 }
                 if ((fl & F_PID) && (mask & (1u << T_PID))) {
-                    int i1 = owner ? s : j, i2 = owner ? j : s;
-                    v3<double> p1 = owner ? ri : rj, p2 = owner ? rj : ri;
-                    v3<double> p1p = ld3(Pp, n, i1 - 1), p1n = ld3(Pp, n, i1 + 1);
-                    v3<double> p2p = ld3(Pp, n, i2 - 1), p2n = ld3(Pp, n, i2 + 1);
-                    v3<double> r12 = owner ? dd : -dd;
-                    int ty = 20 * D.atype[i1] + D.atype[i2];
+                    const v3<double> rjp = ld3(Pp, n, j - 1), rjn = ld3(Pp, n, j + 1);
+                    const v3<double> p1 = owner ? ri : rj, p2 = owner ? rj : ri;
+                    const v3<double> p1p = owner ? rsp : rjp, p1n = owner ? rsn : rjn;
+                    const v3<double> p2p = owner ? rjp : rsp, p2n = owner ? rjn : rsn;
+                    const v3<double> r12 = owner ? dd : -dd;
+                    const int i1 = owner ? s : j, i2 = owner ? j : s;
+                    const int ty = 20 * D.atype[i1] + D.atype[i2];
                     v3<T> out[6];
                     T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], cvt<T>(p1 - p1p),
                                         cvt<T>(p1n - p1), cvt<T>(p1n - p1p), cvt<T>(r12), cvt<T>(p2 - p2p),
                                         cvt<T>(p2n - p2), cvt<T>(p2n - p2p), cvt<T>(-r12), cvt<T>(r12), out);
-                    int o = owner ? 0 : 3;
+                    const int o = owner ? 0 : 3;
                     add3(Fx, Fy, Fz, out[o + 1]);
                     add3(tpx, tpy, tpz, out[o]);
                     add3(tnx, tny, tnz, out[o + 2]);
@@ -635,8 +640,8 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
         }
         // quasi-adiabatic contacts through the bead's contact slots
         if (mask & (1u << T_QA)) {
-            int q = D.qn[s], tot = D.ct_cap[D.nrep];
-            for (int k = 0; k < q; ++k) {
+            const int q = D.qn[s], tot = D.ct_cap[D.nrep];
+            for (int k = sub; k < q; k += LPB) {
                 int ci = D.qslot[k * n + s];
                 double fx = D.ct_F[ci], fy = D.ct_F[tot + ci], fz = D.ct_F[2 * tot + ci];
                 if (D.ct_i1[ci] == s) {
@@ -652,25 +657,32 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
             }
         }
     }
-    if constexpr (EVAL) {
-        if (active)
+    Fx = lane_sum<LPB>(Fx);
+    Fy = lane_sum<LPB>(Fy);
+    Fz = lane_sum<LPB>(Fz);
+    V = lane_sum<LPB>(V);
+    tpx = lane_sum<LPB>(tpx);
+    tpy = lane_sum<LPB>(tpy);
+    tpz = lane_sum<LPB>(tpz);
+    tnx = lane_sum<LPB>(tnx);
+    tny = lane_sum<LPB>(tny);
+    tnz = lane_sum<LPB>(tnz);
+    if constexpr (EVAL)
+        for (int k = 0; k < NUM_TERMS; ++k) Vt[k] = lane_sum<LPB>(Vt[k]);
+    if (active && sub == 0) {
+        if constexpr (EVAL)
             for (int k = 0; k < NUM_TERMS; ++k) D.Vterm[k * n + s] = Vt[k];
-    }
-    if constexpr (!FUSED) {
-        if (active) {
-            D.Fbuf[s] = Fx;
-            D.Fbuf[n + s] = Fy;
-            D.Fbuf[2 * n + s] = Fz;
-            D.Vbead[s] = V;
-            double *ps = D.pid_side;
-            ps[s] = tpx; ps[n + s] = tpy; ps[2 * n + s] = tpz;
-            ps[3 * n + s] = tnx; ps[4 * n + s] = tny; ps[5 * n + s] = tnz;
-        }
-        return;
-    } else {
-        if (active) D.Vbead[s] = V;
-        // integrate (whole wavefront participates in the RNG jump)
-        integrate_bead_impl(D, G, p, s, rep, active, Fx, Fy, Fz);
+        D.Fbuf[s] = Fx;
+        D.Fbuf[n + s] = Fy;
+        D.Fbuf[2 * n + s] = Fz;
+        D.Vbead[s] = V;
+        double *ps = D.pid_side;
+        ps[s] = tpx;
+        ps[n + s] = tpy;
+        ps[2 * n + s] = tpz;
+        ps[3 * n + s] = tnx;
+        ps[4 * n + s] = tny;
+        ps[5 * n + s] = tnz;
     }
 }
 
@@ -732,9 +744,7 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
     int l = lane_id();
     if (rep == 0 && l == 0) D.any_rb[1 - p] = 0;
     if (mode == 1) {
-        int nb = D.rep_n[rep];
-        uint64_t S = wave_lane_state(D, D.rng[p][rep], nb >> 6);
-        uint64_t Sn = __shfl(S, nb & 63);
+        uint64_t Sn = wave_matvec_rows(D.jadv + 64 * rep, D.rng[p][rep]);
         if (l == 0) {
             D.rng[1 - p][rep] = Sn;
             D.tt[1 - p][rep] = D.tt[p][rep] + G.dt;
@@ -807,7 +817,16 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
     for (int c0 = lo; c0 < hi; c0 += 1024) {
         int c = c0 + tid;
         int v = 0;
-        if (c < hi) v = mode == 0 ? D.cell_cnt[c] : (mode == 1 ? 64 * D.slice_w[c] : D.fp_slice_cnt[c]);
+        if (c < hi) {
+            if (mode == 0) v = D.cell_cnt[c];
+            else if (mode == 2) v = D.fp_slice_cnt[c];
+            else {
+                int wmax = 0;
+                for (int q = 0; q < 64; ++q) wmax = max(wmax, D.nl_len[64 * c + q]);
+                D.slice_w[c] = wmax;
+                v = 64 * wmax;
+            }
+        }
         s_buf[tid] = v;
         __syncthreads();
         for (int sh = 1; sh < 1024; sh <<= 1) {
@@ -853,30 +872,6 @@ __global__ __launch_bounds__(BLK) void k_cell_fill(Dev D, int p) {
     D.cell_beads[k] = s;
 }
 
-// Visit every candidate partner j of slot s (cells or the whole replica).
-template <typename Fn> __device__ inline void for_candidates(const Dev &D, int s, int rep, Fn fn) {
-    int c0 = D.cell_off[rep];
-    if (D.cell_off[rep + 1] == c0) {
-        int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
-        for (int j = a; j < b; ++j)
-            if (j != s) fn(j);
-        return;
-    }
-    const int *nd = D.cell_dim + 3 * rep;
-    int c = D.bead_cell[s] - c0;
-    int cx = c % nd[0], cy = (c / nd[0]) % nd[1], cz = c / (nd[0] * nd[1]);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int x = (cx + dx + nd[0]) % nd[0], y = (cy + dy + nd[1]) % nd[1], z = (cz + dz + nd[2]) % nd[2];
-                int cc = c0 + (z * nd[1] + y) * nd[0] + x;
-                for (int k = D.cell_start[cc]; k < D.cell_end[cc]; ++k) {
-                    int j = D.cell_beads[k];
-                    if (j != s) fn(j);
-                }
-            }
-}
-
 // Row flags of the pair (s, j) at rebuild: legacy_update's criterion
 // (legacy_update.cpp:24-29) and the per-term filters of update_X
 // (pauli.cpp:13-20, es_pair.cpp:9-21, bundle.cpp:9-26, free_pair.cpp:9-19).
@@ -915,75 +910,141 @@ __device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &
     return f;
 }
 
-__global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double *P, int p) {
-    if (!D.any_rb[p]) return;
-    int s = blockIdx.x * BLK + threadIdx.x;
-    int rep = D.slot_rep[s];
-    if (!D.rb_now[rep]) return;  // wave-uniform: waves never span replicas
-    int cnt = 0;
-    if (D.bflags[s] & B_ACTIVE) {
-        Box box = D.box[rep];
-        v3<double> rs = ld3(P, D.nslot, s);
-        for_candidates(D, s, rep, [&](int j) {
-            uint8_t f = pair_flags(D, G, box, s, j, rs, P);
-            if (f & (F_PAULI | F_ES | F_PID | F_QAFREE)) ++cnt;
-        });
+// Wavefront per slot: the lanes share the slot's candidate partners.
+template <typename Fn> __device__ inline void for_candidates_wave(const Dev &D, int s, int rep, Fn fn) {
+    const int l = lane_id();
+    int c0 = D.cell_off[rep];
+    if (D.cell_off[rep + 1] == c0) {
+        int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
+        for (int j0 = a; j0 < b; j0 += WAVE) {
+            int j = j0 + l;
+            fn(j < b && j != s ? j : -1);
+        }
+        return;
     }
-    D.nl_len[s] = cnt;
-    int m = cnt;
-    for (int o = 32; o > 0; o >>= 1) {
-        int x = __shfl_xor(m, o);
-        m = x > m ? x : m;
-    }
-    if (lane_id() == 0) D.slice_w[s >> 6] = m;
+    const int *nd = D.cell_dim + 3 * rep;
+    int c = D.bead_cell[s] - c0;
+    int cx = c % nd[0], cy = (c / nd[0]) % nd[1], cz = c / (nd[0] * nd[1]);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = (cx + dx + nd[0]) % nd[0], y = (cy + dy + nd[1]) % nd[1], z = (cz + dz + nd[2]) % nd[2];
+                int cc = c0 + (z * nd[1] + y) * nd[0] + x;
+                int a = D.cell_start[cc], b = D.cell_end[cc];
+                for (int k0 = a; k0 < b; k0 += WAVE) {
+                    int k = k0 + l;
+                    int j = k < b ? D.cell_beads[k] : -1;
+                    fn(j != s ? j : -1);
+                }
+            }
 }
 
+constexpr int NL_WAVES = BLK / WAVE;  // slots per block in the rebuild kernels
+constexpr int MAXSORT = 512;
+
+// Row length per slot (entries carrying any term flag).
+__global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double *P, int p) {
+    if (!D.any_rb[p]) return;
+    const int s = blockIdx.x * NL_WAVES + threadIdx.x / WAVE;
+    const int rep = D.slot_rep[s];
+    if (!D.rb_now[rep]) return;  // uniform: the block's slots share a replica
+    int cnt = 0;
+    if (D.bflags[s] & B_ACTIVE) {
+        const Box box = D.box[rep];
+        const v3<double> rs = ld3(P, D.nslot, s);
+        for_candidates_wave(D, s, rep, [&](int j) {
+            if (j >= 0 && (pair_flags(D, G, box, s, j, rs, P) & (F_PAULI | F_ES | F_PID | F_QAFREE))) ++cnt;
+        });
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (lane_id() == 0) {
+        D.nl_len[s] = cnt;
+        if ((s & 63) == 0) D.fp_slice_cnt[s >> 6] = 0;
+    }
+}
+
+// Fill each row in LDS, rank-sort it by partner (partners are distinct),
+// write it; count the half row and the QA free pairs; snapshot orig_r and
+// the Go activity.  Rows longer than MAXSORT are written unsorted and
+// insertion-sorted in place by lane 0.
 __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double *P, int p) {
+    __shared__ int s_j[NL_WAVES][MAXSORT];
+    __shared__ uint8_t s_f[NL_WAVES][MAXSORT];
     if (!D.any_rb[p]) return;
     if (*D.err & E_NL_CAP) return;
-    int s = blockIdx.x * BLK + threadIdx.x;
-    int rep = D.slot_rep[s];
-    if (!D.rb_now[rep]) return;
+    const int w = threadIdx.x / WAVE, l = lane_id();
+    const int s = blockIdx.x * NL_WAVES + w;
+    const int rep = D.slot_rep[s];
+    if (!D.rb_now[rep]) return;  // uniform: the block's slots share a replica
+    const bool active = D.bflags[s] & B_ACTIVE;
     const int n = D.nslot;
-    int nfp = 0;
-    if (D.bflags[s] & B_ACTIVE) {
-        Box box = D.box[rep];
-        v3<double> rs = ld3(P, n, s);
-        int base = D.slice_start[s >> 6] + (s & 63);
-        int cnt = 0;
-        for_candidates(D, s, rep, [&](int j) {
-            uint8_t f = pair_flags(D, G, box, s, j, rs, P);
-            if (f & (F_PAULI | F_ES | F_PID | F_QAFREE)) {
-                // insertion into the sorted row
-                int k = cnt++;
-                while (k > 0 && D.nl_j[base + 64 * (k - 1)] > j) {
-                    D.nl_j[base + 64 * k] = D.nl_j[base + 64 * (k - 1)];
-                    D.nl_flag[base + 64 * k] = D.nl_flag[base + 64 * (k - 1)];
-                    --k;
+    const Box box = D.box[rep];
+    const v3<double> rs = ld3(P, n, s);
+    const int base = D.slice_start[s >> 6] + (s & 63);
+    const int len = active ? D.nl_len[s] : 0;
+    const bool in_lds = len <= MAXSORT;
+    int cnt = 0, nfp = 0;
+    if (active) {
+        for_candidates_wave(D, s, rep, [&](int j) {
+            uint8_t f = j >= 0 ? pair_flags(D, G, box, s, j, rs, P) : 0;
+            bool keep = f & (F_PAULI | F_ES | F_PID | F_QAFREE);
+            unsigned long long m = __ballot(keep);
+            int pos = cnt + __popcll(m & ((1ull << l) - 1ull));
+            if (keep) {
+                if (in_lds) {
+                    s_j[w][pos] = j;
+                    s_f[w][pos] = f;
+                } else {
+                    D.nl_j[base + 64 * pos] = j;
+                    D.nl_flag[base + 64 * pos] = f;
                 }
-                D.nl_j[base + 64 * k] = j;
-                D.nl_flag[base + 64 * k] = f;
+                nfp += (f & F_QAFREE) != 0;
             }
+            cnt += __popcll(m);
         });
-        int half = 0;
-        for (int k = 0; k < cnt; ++k) {
-            int j = D.nl_j[base + 64 * k];
-            if (j < s) half = k + 1;
-            if (D.nl_flag[base + 64 * k] & F_QAFREE) ++nfp;
-        }
-        D.nl_half[s] = half;
-        // snapshot for verify, Go activity (go.cpp:11-22)
-        st3(D.orig_r, n, s, rs);
-        for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k) {
-            int j = D.nat_j[k];
-            v3<double> d = wrap(box, ld3(P, n, j) - rs);
-            double cutoff = 2.0 * D.nat_d[k];
-            D.nat_act[k] = norm(d) < cutoff + G.pad;
-        }
     }
-    int sum = nfp;
-    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-    if (lane_id() == 0) D.fp_slice_cnt[s >> 6] = sum;
+    __syncthreads();
+    int below = 0;
+    if (active && in_lds) {
+        for (int k = l; k < cnt; k += WAVE) {
+            int j = s_j[w][k], r = 0;
+            for (int q = 0; q < cnt; ++q) r += s_j[w][q] < j;
+            D.nl_j[base + 64 * r] = j;
+            D.nl_flag[base + 64 * r] = s_f[w][k];
+            below += j < s;
+        }
+    } else if (active && l == 0) {
+        for (int a = 1; a < cnt; ++a) {
+            int j = D.nl_j[base + 64 * a];
+            uint8_t f = D.nl_flag[base + 64 * a];
+            int k = a;
+            while (k > 0 && D.nl_j[base + 64 * (k - 1)] > j) {
+                D.nl_j[base + 64 * k] = D.nl_j[base + 64 * (k - 1)];
+                D.nl_flag[base + 64 * k] = D.nl_flag[base + 64 * (k - 1)];
+                --k;
+            }
+            D.nl_j[base + 64 * k] = j;
+            D.nl_flag[base + 64 * k] = f;
+        }
+        for (int k = 0; k < cnt; ++k) below += D.nl_j[base + 64 * k] < s;
+    }
+    if (!active) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        below += __shfl_xor(below, o);
+        nfp += __shfl_xor(nfp, o);
+    }
+    if (l == 0) {
+        D.nl_half[s] = below;
+        if (nfp) atomicAdd(D.fp_slice_cnt + (s >> 6), nfp);
+        st3(D.orig_r, n, s, rs);
+    }
+    // Go activity (go.cpp:11-22)
+    for (int k = D.nat_start[s] + l; k < D.nat_start[s + 1]; k += WAVE) {
+        int j = D.nat_j[k];
+        v3<double> d = wrap(box, ld3(P, n, j) - rs);
+        double cutoff = 2.0 * D.nat_d[k];
+        D.nat_act[k] = norm(d) < cutoff + G.pad;
+    }
 }
 
 // Free pairs in canonical order; zero the cell counters; count the rebuild.

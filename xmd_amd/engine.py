@@ -94,3 +94,15 @@ def profile(engine, nsteps):
                                     ms.ctypes.data_as(C.POINTER(C.c_double)), C.c_int32(cap), C.byref(nk)))
     lst = names.value.decode().split("\n")
     return {lst[k]: (int(counts[k]), float(ms[k])) for k in range(nk.value)}
+
+
+def first_zero_log_draw(state, n_beads, n_steps):
+    """(step, bead) of the first zero uniform draw that feeds log() in
+    rand_gen's Langevin noise within n_steps, or None."""
+    L = lib()
+    L.xmd_rng_first_zero_draw.restype = C.c_int64
+    L.xmd_rng_first_zero_draw.argtypes = [C.c_uint64, C.c_int64, C.c_uint32]
+    idx = L.xmd_rng_first_zero_draw(C.c_uint64(int(state)), C.c_int64(4 * n_beads * n_steps), C.c_uint32(5))
+    if idx < 0:
+        return None
+    return idx // (4 * n_beads), (idx // 4) % n_beads

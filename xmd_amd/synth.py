@@ -277,7 +277,22 @@ def config_params(cid, precision=_abi.PREC_FP64, rng_exact=True):
     return p
 
 
-def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, seed=None):
+def _pick_seeds(start, count, n_beads_of, horizon):
+    """`count` seeds from `start` upward, skipping any whose rand_gen stream
+    feeds a zero uniform draw to log() within `horizon` steps: that draw
+    makes the reference's Langevin noise infinite (rand_gen.h:45-46)."""
+    if horizon <= 0:
+        return [start + k for k in range(count)]
+    from .engine import first_zero_log_draw
+    seeds, cand = [], start
+    while len(seeds) < count:
+        if first_zero_log_draw(cand, n_beads_of(len(seeds)), horizon) is None:
+            seeds.append(cand)
+        cand += 1
+    return seeds
+
+
+def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, seed=None, horizon_steps=0):
     """Synthetic system of configuration cid.
 
     C0: 1 x 60, open box.  C1: 1 x 1000, open box.
@@ -285,36 +300,38 @@ def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, 
     C3: 512 single chains, lengths U{30..400}, T ~ U[0.30, 0.50].
     C4: 400 x 250 at 5e-4 residues/A^3 (L = 584.8 A).
     n_batch > 1 repeats C2/C4 as independent boxes with temperatures
-    0.30 + 0.2 k/(n_batch-1) and seeds base + k.
+    0.30 + 0.2 k/(n_batch-1) and seeds base + k.  With horizon_steps > 0,
+    seeds whose noise stream hits a zero log() draw within that many steps
+    are skipped (the next seed is taken).
     """
     base = SEED_BASE + cid if seed is None else seed
     reps = []
     if cid in (0, 1):
         n = {0: 60, 1: 1000}[cid] if chain_len is None else chain_len
-        for k in range(n_batch):
-            rng = np.random.default_rng(base + k)
-            reps.append(dict(chains=[random_chain(n, rng)], box_A=(0, 0, 0), temperature=0.35, seed=base + k))
+        for k, sd in enumerate(_pick_seeds(base, n_batch, lambda _: n, horizon_steps)):
+            rng = np.random.default_rng(sd)
+            reps.append(dict(chains=[random_chain(n, rng)], box_A=(0, 0, 0), temperature=0.35, seed=sd))
     elif cid in (2, 4):
         nch = {2: 40, 4: 400}[cid] if chains is None else chains
         m = 250 if chain_len is None else chain_len
         dens = {2: 1e-4, 4: 5e-4}[cid]
         L = (nch * m / dens) ** (1.0 / 3.0)
-        for k in range(n_batch):
-            rng = np.random.default_rng(base + k)
+        for k, sd in enumerate(_pick_seeds(base, n_batch, lambda _: nch * m, horizon_steps)):
+            rng = np.random.default_rng(sd)
             chs = [random_chain(m, rng) for _ in range(nch)]
             placed = place_chains([c[0] for c in chs], L, rng)
             T = 0.35 if n_batch == 1 else 0.30 + 0.2 * k / (n_batch - 1)
             reps.append(dict(chains=[(p, c[1]) for p, c in zip(placed, chs)], box_A=(L, L, L),
-                             temperature=T, seed=base + k))
+                             temperature=T, seed=sd))
     elif cid == 3:
         nrep = 512 if n_replicas is None else n_replicas
         rng0 = np.random.default_rng(base)
         lens = rng0.integers(30, 401, size=nrep)
         temps = rng0.uniform(0.30, 0.50, size=nrep)
-        for k in range(nrep):
-            rng = np.random.default_rng(base + 1 + k)
+        for k, sd in enumerate(_pick_seeds(base + 1, nrep, lambda i: int(lens[i]), horizon_steps)):
+            rng = np.random.default_rng(sd)
             reps.append(dict(chains=[random_chain(int(lens[k]), rng)], box_A=(0, 0, 0),
-                             temperature=float(temps[k]), seed=base + 1 + k))
+                             temperature=float(temps[k]), seed=sd))
     else:
         raise ValueError(cid)
     return make_system(reps)
