@@ -64,6 +64,20 @@ def force_kernel_bytes(eng, system, fused):
     return 5 * e + 13 * nnat + per_bead * n
 
 
+def copy_calibration(mib):
+    """Known-traffic kernels: a D2D copy and out = 2*x, each reading and
+    writing `mib` MiB once."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+        torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "read_bytes": n * 4, "write_bytes": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,11 +88,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy and a scale kernel of MIB MiB each "
+                         "(known HBM bytes, to calibrate FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed "
                                    "(independent replicas per GPU) but not built in this version"}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     t0 = time.time()
     # seeds whose noise would hit a zero log() draw inside the run are

@@ -347,6 +347,8 @@ void derive(xmd_engine &E, const xmd_topology *t) {
     G.half_dt2 = p.dt * p.dt / 2.0;
     G.gamma = p.gamma_factor * 1.0;
     G.lang_on = p.langevin_enabled != 0;
+    G.H1 = p.tether_H1;
+    G.H2 = p.tether_H2;
     double bb_rmin = p.lj_bb_rmin, bs_rmin = p.lj_bs_rmin;
     std::vector<double> ss_rmax(p.ss_rmax, p.ss_rmax + XMD_NUM_SS);
     double es_cut = p.const_es_enabled ? 2.0 * p.const_es_screen : 2.0 * p.rel_es_screen;

@@ -35,6 +35,7 @@ enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_C
 // Derived parameters in double (list and QA decisions, integrator).
 struct Derived {
     double dt, dt_inv, gamma, half_dt2;
+    double H1, H2;  // tether constants (FP64 in both forms)
     double pad, req_r;
     double pauli_min_inv, es_min_inv, pid_min_inv, qa_min_inv;
     int pauli_on, es_on, pid_on, qa_on, go_on;
@@ -491,6 +492,21 @@ __device__ inline void add3(double &x, double &y, double &z, v3<T> f) {
     z += (double)f.z;
 }
 
+// Element k (runtime) of a small local array by selects on register values.
+// The empty asm statements pin each element in registers first; otherwise
+// the selects fold into an indexed load and the array moves to scratch.
+template <typename T> __device__ inline v3<T> in_regs(v3<T> v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
+    return v;
+}
+template <typename T> __device__ inline v3<T> pick3(const v3<T> a[3], int k) {
+    const v3<T> a0 = in_regs(a[0]), a1 = in_regs(a[1]), a2 = in_regs(a[2]);
+    return k == 0 ? a0 : (k == 1 ? a1 : a2);
+}
+template <typename T> __device__ inline v3<T> pick4(const v3<T> a[4], int k) {
+    const v3<T> a0 = in_regs(a[0]), a1 = in_regs(a[1]), a2 = in_regs(a[2]), a3 = in_regs(a[3]);
+    return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+}
 template <int LPB> __device__ inline double lane_sum(double v) {
     for (int o = LPB / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
@@ -535,7 +551,7 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
                     double nc = D.chir_nat[first];
                     if (nc == nc) {
                         T v = chiral<T>(r12, r23, r34, P.e_chi, (T)nc, (T)D.chir_fac[first], out);
-                        add3(Fx, Fy, Fz, out[3 - a]);
+                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
                         if (a == 3) addV(T_CHIRAL, (double)v);
                     }
                 }
@@ -543,12 +559,12 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
                 if (ph == ph) {
                     if (mask & (1u << T_CND)) {
                         T v = nat_dihedral<T>(r12, r23, r34, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
-                        add3(Fx, Fy, Fz, out[3 - a]);
+                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
                         if (a == 3) addV(T_CND, (double)v);
                     }
                     if (mask & (1u << T_SND)) {
                         T v = nat_dihedral<T>(r12, r23, r34, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
-                        add3(Fx, Fy, Fz, out[3 - a]);
+                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
                         if (a == 3) addV(T_SND, (double)v);
                     }
                 }
@@ -561,19 +577,20 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
                 v3<double> q0 = ld3(Pp, n, first), q1 = ld3(Pp, n, first + 1), q2 = ld3(Pp, n, first + 2);
                 v3<T> out[3];
                 T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
-                add3(Fx, Fy, Fz, out[2 - a]);
+                add3(Fx, Fy, Fz, pick3(out, 2 - a));
                 if (a == 2) addV(T_NAT_ANG, (double)v);
             } else {
+                // FP64 in both forms: stiff bonds cancel in r - r_nat
                 if (!(mask & (1u << T_TETHER))) continue;
-                v3<T> out[2];
+                v3<double> out[2];
                 if (bf & B_PREV) {
-                    tether<T>(cvt<T>(ri - ld3(Pp, n, s - 1)), P.H1, P.H2, (T)D.tether_nat[s - 1], out);
+                    tether<double>(ri - ld3(Pp, n, s - 1), G.H1, G.H2, D.tether_nat[s - 1], out);
                     add3(Fx, Fy, Fz, out[1]);
                 }
                 if (bf & B_NEXT) {
-                    T v = tether<T>(cvt<T>(ld3(Pp, n, s + 1) - ri), P.H1, P.H2, (T)D.tether_nat[s], out);
+                    double v = tether<double>(ld3(Pp, n, s + 1) - ri, G.H1, G.H2, D.tether_nat[s], out);
                     add3(Fx, Fy, Fz, out[0]);
-                    addV(T_TETHER, (double)v);
+                    addV(T_TETHER, v);
                 }
             }
         }
@@ -619,21 +636,20 @@ __global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, 
                     if (owner) addV(es_term, (double)f.V);
                 }
                 if ((fl & F_PID) && (mask & (1u << T_PID))) {
+                    // The bundle from this bead's side: s in the first slot.
+                    // lambda(psi_1) lambda(psi_2) LJ(r) is symmetric in the
+                    // two sides, so out[0..2] are the reference's forces on
+                    // (s-1, s, s+1) whichever bead of the pair is i1; the
+                    // energy is counted by the owner (s = i1).
                     const v3<double> rjp = ld3(Pp, n, j - 1), rjn = ld3(Pp, n, j + 1);
-                    const v3<double> p1 = owner ? ri : rj, p2 = owner ? rj : ri;
-                    const v3<double> p1p = owner ? rsp : rjp, p1n = owner ? rsn : rjn;
-                    const v3<double> p2p = owner ? rjp : rsp, p2n = owner ? rjn : rsn;
-                    const v3<double> r12 = owner ? dd : -dd;
                     const int i1 = owner ? s : j, i2 = owner ? j : s;
                     const int ty = 20 * D.atype[i1] + D.atype[i2];
                     v3<T> out[6];
-                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], cvt<T>(p1 - p1p),
-                                        cvt<T>(p1n - p1), cvt<T>(p1n - p1p), cvt<T>(r12), cvt<T>(p2 - p2p),
-                                        cvt<T>(p2n - p2), cvt<T>(p2n - p2p), cvt<T>(-r12), cvt<T>(r12), out);
-                    const int o = owner ? 0 : 3;
-                    add3(Fx, Fy, Fz, out[o + 1]);
-                    add3(tpx, tpy, tpz, out[o]);
-                    add3(tnx, tny, tnz, out[o + 2]);
+                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], ri - rsp, rsn - ri,
+                                        rsn - rsp, dd, rj - rjp, rjn - rj, rjn - rjp, -dd, dd, out);
+                    add3(Fx, Fy, Fz, out[1]);
+                    add3(tpx, tpy, tpz, out[0]);
+                    add3(tnx, tny, tnz, out[2]);
                     if (owner) addV(T_PID, (double)v);
                 }
             }

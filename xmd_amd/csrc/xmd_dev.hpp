@@ -12,6 +12,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "bm_math.hpp"
 
 namespace xmd_dev {
@@ -28,8 +30,44 @@ template <typename T> XMD_HD inline v3<T> operator*(T s, v3<T> a) { return {s * 
 template <typename T> XMD_HD inline v3<T> operator/(v3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
 template <typename T> XMD_HD inline T dot(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <typename T> XMD_HD inline T norm_sq(v3<T> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
-template <typename T> XMD_HD inline T norm(v3<T> a) { return sqrt(norm_sq(a)); }
-template <typename T> XMD_HD inline T norm_inv(v3<T> a) { return T(1.0) / norm(a); }
+// FP32 reciprocal, square root and reciprocal square root: the hardware
+// approximations on the device (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about
+// 1 ulp; every argument here is a positive normal number), correctly
+// rounded on the host.  The FP64 form keeps correctly rounded operations.
+XMD_HD inline float rcp_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+XMD_HD inline float sqrt_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+XMD_HD inline float rsq_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+template <typename T> XMD_HD inline T rcpT(T x) {
+    if constexpr (std::is_same<T, float>::value) return rcp_f(x);
+    else return T(1.0) / x;
+}
+template <typename T> XMD_HD inline T sqrtT(T x) {
+    if constexpr (std::is_same<T, float>::value) return sqrt_f(x);
+    else return sqrt(x);
+}
+template <typename T> XMD_HD inline T norm(v3<T> a) { return sqrtT(norm_sq(a)); }
+template <typename T> XMD_HD inline T norm_inv(v3<T> a) {
+    if constexpr (std::is_same<T, float>::value) return rsq_f(norm_sq(a));
+    else return T(1.0) / norm(a);
+}
 template <typename T> XMD_HD inline v3<T> cross(v3<T> a, v3<T> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
@@ -41,6 +79,49 @@ template <typename T> XMD_HD inline T acos_c(T x) {  // xmd/include/xmd/utils/ma
     return acos(x_);
 }
 template <typename T, typename U> XMD_HD inline v3<T> cvt(v3<U> a) { return {(T)a.x, (T)a.y, (T)a.z}; }
+
+// FP32 atan2: atan(a) = a P(a^2) on a = min/max of |x|, |y| in [0, 1]
+// (degree-8 polynomial in a^2 fitted by iteratively reweighted least
+// squares on Chebyshev nodes; max error 6e-9 in exact arithmetic, about
+// 1 ulp evaluated in FP32), then the octant reflections.
+XMD_HD inline float atan2_f(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.0f ? mn * rcp_f(mx) : 0.0f, z = a * a;
+    float p = 0.0024567232467234135f;
+    p = p * z - 0.014401356689631939f;
+    p = p * z + 0.03978123143315315f;
+    p = p * z - 0.07234858721494675f;
+    p = p * z + 0.10498947650194168f;
+    p = p * z - 0.14161230623722076f;
+    p = p * z + 0.19985906779766083f;
+    p = p * z - 0.33332598209381104f;
+    p = p * z + 0.9999998807907104f;
+    float r = p * a;
+    if (ay > ax) r = 1.5707963267948966f - r;
+    if (x < 0.0f) r = 3.141592653589793f - r;
+    return copysignf(r, y);
+}
+// sin and cos of h, |h| <= pi/2, by their Taylor series (truncation error
+// below 7e-10 there).
+XMD_HD inline void sincos_half_f(float h, float &sh, float &ch) {
+    const float z = h * h;
+    float ps = 1.0f / 6227020800.0f;
+    ps = ps * z - 1.0f / 39916800.0f;
+    ps = ps * z + 1.0f / 362880.0f;
+    ps = ps * z - 1.0f / 5040.0f;
+    ps = ps * z + 1.0f / 120.0f;
+    ps = ps * z - 1.0f / 6.0f;
+    sh = h + h * (ps * z);
+    float pc = -1.0f / 87178291200.0f;
+    pc = pc * z + 1.0f / 479001600.0f;
+    pc = pc * z - 1.0f / 3628800.0f;
+    pc = pc * z + 1.0f / 40320.0f;
+    pc = pc * z - 1.0f / 720.0f;
+    pc = pc * z + 1.0f / 24.0f;
+    pc = pc * z - 0.5f;
+    ch = 1.0f + pc * z;
+}
 
 // Orthorhombic box, xmd/include/xmd/model/box.h:36-49 (cell_inv 0 = open).
 struct Box {
@@ -92,12 +173,20 @@ template <typename T> XMD_HD inline bool lam_supp(T psi0, T alpha, T psi) {
 template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, int version, T psi) {
     T s = alpha * (psi - psi0);
     if (version == 0) {
-        T val = T(0.5f) * cos(s) + T(0.5f);
-        T deriv = T(-0.5f) * alpha * sin(s);
-        return {val, deriv};
+        if constexpr (std::is_same<T, float>::value) {
+            // the same function through the half angle (|s| < pi on the
+            // support): (1 + cos s)/2 = cos^2(s/2), sin s = 2 sin(s/2) cos(s/2)
+            float sh, ch;
+            sincos_half_f(0.5f * s, sh, ch);
+            return {ch * ch, -alpha * sh * ch};
+        } else {
+            T val = T(0.5f) * cos(s) + T(0.5f);
+            T deriv = T(-0.5f) * alpha * sin(s);
+            return {val, deriv};
+        }
     }
     T t = fabs(s / T(M_PI));
-    T x_inv = T(1.0f) / (T(2.0f) * t * t - T(2.0f) * t - T(1));
+    T x_inv = rcpT(T(2.0f) * t * t - T(2.0f) * t - T(1));
     T val = (t * t - T(2.0f) * t + T(1.0f)) * x_inv;
     T deriv = T(2.0f) * (t - T(1.0f)) * (t - T(2.0f)) * x_inv * x_inv * (alpha / T(M_PI));
     deriv *= (s < T(0.0f) ? T(-1.0f) : T(1.0f));
@@ -145,7 +234,7 @@ template <typename T> XMD_HD inline FV<T> pauli_pair(v3<T> d, T depth, T r_excl)
 }
 // es/const.cpp:13-32 (+D9) and es/relative.cpp:13-32
 template <typename T> XMD_HD inline FV<T> es_pair(v3<T> d, T q1q2, T Vf, T sdi, int relative) {
-    T r = norm(d), rn = T(1.0f) / r;
+    T r = norm(d), rn = rcpT(r);
     v3<T> u = d * rn;
     T V, dV;
     if (relative) {
@@ -159,39 +248,53 @@ template <typename T> XMD_HD inline FV<T> es_pair(v3<T> d, T q1q2, T Vf, T sdi, 
 }
 
 // One pseudo-improper dihedral (eval_forces.cpp:27-57) with the gradient
-// of psi (D8).  Inputs: r12 = cen - prev, r23 = next - cen, r13 = next -
-// prev, r24 = wrap(partner - cen).
+// of psi (D8).  Inputs (FP64): r12 = cen - prev, r23 = next - cen, r13 =
+// next - prev, r24 = wrap(partner - cen).  The plane normals (cross
+// products of a bond with a vector up to the list radius, which cancel)
+// are formed in FP64 in both forms and narrowed; the FP32 form then takes
+// psi = atan2 of its sine and cosine parts (well conditioned) and forms
+// the gradient in FP32.
 template <typename T> struct Psi {
     T psi;
     v3<T> g_prev, g_cen, g_next, g_par;
 };
-template <typename T> XMD_HD inline Psi<T> pid_psi(v3<T> r12, v3<T> r23, v3<T> r13, v3<T> r24) {
-    v3<T> r14 = r12 + r24;
-    v3<T> rij = -r23, rkj = -r13, rkl = -r14;
-    v3<T> rm = cross(r23, r12);
-    v3<T> rn = cross(r13, r14);
-    T rm_rn = norm_inv(rm), rn_rn = norm_inv(rn);
-    T rkj_n = norm(rkj);
-    T cos_psi = dot(rm, rn) * rm_rn * rn_rn;
-    T psi = acos_c(cos_psi);
-    if (dot(rij, rn) < T(0.0f)) psi = -psi;
-    v3<T> gi = rm * (rkj_n * rm_rn * rm_rn);
-    v3<T> gl = -(rn * (rkj_n * rn_rn * rn_rn));
-    T inv2 = T(1) / (rkj_n * rkj_n);
-    T p = dot(rij, rkj) * inv2, q = dot(rkl, rkj) * inv2;
-    v3<T> gj = (p - T(1)) * gi - q * gl;
-    v3<T> gk = (q - T(1)) * gl - p * gi;
+template <typename T>
+XMD_HD inline Psi<T> pid_psi(v3<double> r12d, v3<double> r23d, v3<double> r13d, v3<double> r24d) {
+    const v3<double> r14d = r12d + r24d;
+    const v3<double> rmd = cross(r23d, r12d), rnd = cross(r13d, r14d);
+    const v3<T> rm = cvt<T>(rmd), rn = cvt<T>(rnd);
+    const v3<T> rij = cvt<T>(-r23d), rkj = cvt<T>(-r13d), rkl = cvt<T>(-r14d);
+    const T rm_rn = norm_inv(rm), rn_rn = norm_inv(rn);
+    const T rkj_n = norm(rkj);
+    T psi;
+    if constexpr (std::is_same<T, double>::value) {
+        double cos_psi = dot(rm, rn) * rm_rn * rn_rn;
+        psi = acos_c(cos_psi);
+        if (dot(rij, rn) < 0.0) psi = -psi;
+    } else {
+        // sin psi |m||n| = (m.r_kl)|r_kj|: (m x n) = (m.r_kl) r_kj, r_ij.n = m.r_kl;
+        // m.r_kl cancels like the cross products, so it is formed in FP64
+        psi = atan2_f((float)dot(rmd, -r14d) * rkj_n, dot(rm, rn));
+    }
+    const v3<T> gi = rm * (rkj_n * rm_rn * rm_rn);
+    const v3<T> gl = -(rn * (rkj_n * rn_rn * rn_rn));
+    const T inv2 = rcpT(rkj_n * rkj_n);
+    const T pp = dot(rij, rkj) * inv2, qq = dot(rkl, rkj) * inv2;
+    const v3<T> gj = (pp - T(1.0f)) * gi - qq * gl;
+    const v3<T> gk = (qq - T(1.0f)) * gl - pp * gi;
     return {psi, gk, gi, gj, gl};
 }
 // Full PID bundle (eval_forces.cpp:93-140, +D8).  Geometry of the canonical
 // pair (i1 < i2).  out[0..5] = forces on i1p, i1, i1n, i2p, i2, i2n.
 template <typename T>
-XMD_HD inline T pid_bundle(const TermParams<T> &P, T ss_depth, T ss_rmin, T ss_rmax, v3<T> a12, v3<T> a23, v3<T> a13,
-                           v3<T> a24, v3<T> b12, v3<T> b23, v3<T> b13, v3<T> b24, v3<T> r12, v3<T> out[6]) {
-    Psi<T> g1 = pid_psi(a12, a23, a13, a24);
-    Psi<T> g2 = pid_psi(b12, b23, b13, b24);
+XMD_HD inline T pid_bundle(const TermParams<T> &P, T ss_depth, T ss_rmin, T ss_rmax, v3<double> a12, v3<double> a23,
+                           v3<double> a13, v3<double> a24, v3<double> b12, v3<double> b23, v3<double> b13,
+                           v3<double> b24, v3<double> r12d, v3<T> out[6]) {
+    Psi<T> g1 = pid_psi<T>(a12, a23, a13, a24);
+    Psi<T> g2 = pid_psi<T>(b12, b23, b13, b24);
     T psi1 = g1.psi, psi2 = g2.psi;
-    T r12_n = norm(r12), r12_rn = T(1.0f) / r12_n;
+    v3<T> r12 = cvt<T>(r12d);
+    T r12_n = norm(r12), r12_rn = rcpT(r12_n);
     T A = 0, B = 0, C = 0, V = 0;
     if (lam_supp(P.bbp_psi0, P.bbp_alpha, psi1) && lam_supp(P.bbp_psi0, P.bbp_alpha, psi2)) {
         VdV<T> l1 = lam_eval(P.bbp_psi0, P.bbp_alpha, P.lambda_version, psi1);
@@ -262,8 +365,13 @@ template <typename T> XMD_HD inline T nat_angle(v3<T> r12, v3<T> r23, T k, T nat
     v3<T> d1 = unit(cross(r12, x12_23)) * r12_rn;
     v3<T> d3 = unit(cross(r23, x12_23)) * r23_rn;
     v3<T> d2 = -d1 - d3;
-    T cos_theta = -dot(r12, r23) * r12_rn * r23_rn;
-    T theta = acos_c(cos_theta);
+    T theta;
+    if constexpr (std::is_same<T, double>::value) {
+        T cos_theta = -dot(r12, r23) * r12_rn * r23_rn;
+        theta = acos_c(cos_theta);
+    } else {
+        theta = atan2_f(norm(x12_23), -dot(r12, r23));  // same angle, FP32-stable
+    }
     T dtheta = theta - nat_theta;
     T V = T(0.5) * k * dtheta * dtheta;
     T dV = k * dtheta;
@@ -278,14 +386,27 @@ XMD_HD inline T nat_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, int kind, T CDA, T
     v3<T> x12_23 = cross(r12, r23), x23_34 = cross(r23, r34);
     T x12_23_rn = norm_inv(x12_23), x23_34_rn = norm_inv(x23_34);
     v3<T> x12_23_u = x12_23 * x12_23_rn, x23_34_u = x23_34 * x23_34_rn;
-    T cos_phi = dot(x12_23_u, x23_34_u);
-    T phi = acos_c(cos_phi);
-    if (dot(x12_23, r34) < T(0.0f)) phi = -phi;
+    T phi;
+    if constexpr (std::is_same<T, double>::value) {
+        T cos_phi = dot(x12_23_u, x23_34_u);
+        phi = acos_c(cos_phi);
+        if (dot(x12_23, r34) < T(0.0f)) phi = -phi;
+    } else {
+        // same angle, FP32-stable: (x12_23 x x23_34) = (x12_23.r34) r23
+        phi = atan2_f(dot(x12_23, r34) * norm(r23), dot(x12_23, x23_34));
+    }
     T diff = phi - nat_phi;
     T V, dV;
     if (kind == 0) {
-        T sin_diff = sin(diff), cos_diff = cos(diff);
-        T sin_3diff = sin(T(3.0f) * diff), cos_3diff = cos(T(3.0f) * diff);
+        T sin_diff, cos_diff, sin_3diff, cos_3diff;
+        if constexpr (std::is_same<T, float>::value) {
+            sincosf(diff, &sin_diff, &cos_diff);  // triple angles from the single one
+            sin_3diff = sin_diff * (3.0f - 4.0f * sin_diff * sin_diff);
+            cos_3diff = cos_diff * (4.0f * cos_diff * cos_diff - 3.0f);
+        } else {
+            sin_diff = sin(diff), cos_diff = cos(diff);
+            sin_3diff = sin(T(3.0f) * diff), cos_3diff = cos(T(3.0f) * diff);
+        }
         V = CDA * (T(1.0f) - cos_diff) + CDB * (T(1.0f) - cos_3diff);
         dV = CDA * sin_diff + CDB * T(3.0f) * sin_3diff;
     } else {
@@ -295,7 +416,10 @@ XMD_HD inline T nat_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, int kind, T CDA, T
     T r23_n = norm(r23);
     v3<T> d1 = -x12_23_u * r23_n * x12_23_rn;
     v3<T> d4 = x23_34_u * r23_n * x23_34_rn;
-    v3<T> df = (-d1 * dot(r12, r23) + d4 * dot(r23, r34)) / (r23_n * r23_n);
+    v3<T> df_num = -d1 * dot(r12, r23) + d4 * dot(r23, r34);
+    v3<T> df;
+    if constexpr (std::is_same<T, float>::value) df = df_num * rcp_f(r23_n * r23_n);
+    else df = df_num / (r23_n * r23_n);
     v3<T> d2 = -d1 + df;
     v3<T> d3 = -d4 - df;
     out[0] = -(dV * d1);
