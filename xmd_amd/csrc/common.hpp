@@ -1,0 +1,118 @@
+// common.hpp -- declarations shared by the engine's two translation units
+// (engine.hip and forces.hip): constants, the derived parameters, the table
+// of device pointers, slot-data access helpers and the force launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "xmd_dev.hpp"
+
+namespace xmd_k {
+using namespace xmd_dev;
+
+constexpr int WAVE = 64;
+constexpr int BLK = 256;
+constexpr int MAXQ = 16;
+constexpr int NUM_TERMS = 11;
+
+enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16 };
+enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
+enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
+enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID };
+
+// Derived parameters in double (list and QA decisions, integrator).
+struct Derived {
+    double dt, dt_inv, gamma, half_dt2;
+    double H1, H2;  // tether constants (FP64 in both forms)
+    double pad, req_r;
+    double pauli_min_inv, es_min_inv, pid_min_inv, qa_min_inv;
+    int pauli_on, es_on, pid_on, qa_on, go_on;
+    int lang_on;
+    QaParams qa;
+};
+
+// Every device pointer of an engine (passed by value to kernels).
+struct Dev {
+    int nslot, nrep;
+    // replicas
+    const int *rep_slot_off, *rep_n;
+    const Box *box;
+    const double *noise_sd;
+    uint64_t *rng[2];
+    double *tt[2];
+    unsigned long long *maxd[2];
+    int *rb_now, *force_rb, *any_rb, *err;
+    long long *n_rebuild;
+    const int *cell_off, *cell_dim;  // [nrep+1], [3*nrep]
+    int *cell_cnt, *cell_start, *cell_end, *cell_cursor, *cell_beads, *bead_cell;
+    // slots
+    const int *slot_rep, *chain;
+    const uint8_t *bflags;
+    const int8_t *atype;
+    double *pos[2];
+    double *y;  // y1..y5, 15*nslot
+    double *orig_r, *Fbuf, *pid_side, *Vbead, *Vterm;
+    const double *tether_nat, *theta_nat, *phi_nat, *chir_nat, *chir_fac;
+    // native contacts (rows of slot s: nat_start[s]..nat_start[s+1])
+    const int *nat_start, *nat_j;
+    const double *nat_d;
+    uint8_t *nat_act;
+    // neighbour rows
+    const int *nl_base, *nl_cap;
+    int *slice_start, *slice_w, *nl_len, *nl_half, *nl_j;
+    uint8_t *nl_flag;
+    // QA free pairs, candidates
+    const int *fp_base, *fp_cap, *fp_chunk_rep;
+    int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt, *fp_slice_start;
+    uint8_t *fp_vac;
+    int *cand_cnt, *cand_fp, *cand_type;
+    uint32_t *cand_d1, *cand_d2;
+    // QA contacts, removals
+    const int *ct_base, *ct_cap, *ct_chunk_rep;
+    int *ct_count, *ct_i1, *ct_i2, *ct_type, *ct_status;
+    double *ct_ref, *ct_F, *ct_V;
+    uint32_t *ct_d1, *ct_d2;
+    uint8_t *ct_vac;
+    int *rm_cnt, *rm_idx;
+    uint32_t *sync;
+    int *qslot, *qn;
+    // tables
+    const double *ss_depth, *ss_rmin, *ss_rmax, *req_min_dist, *charge;
+    const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
+    const int8_t *ptype;
+    const uint8_t *bm_fix;
+    const uint64_t *jrow, *jlane, *jadv;  // jadv: rows of J^(4N) per replica
+};
+
+__device__ inline v3<double> ld3(const double *a, int n, int s) { return {a[s], a[n + s], a[2 * n + s]}; }
+// Positions and the positions at the last list build are AoS, 32 bytes per
+// slot (x, y, z, 0): a partner gather reads one cache line, as a 16-byte
+// and an 8-byte load.
+__device__ inline v3<double> ldp(const double *a, int s) {
+    const double2 xy = *reinterpret_cast<const double2 *>(a + 4 * (size_t)s);
+    return {xy.x, xy.y, a[4 * (size_t)s + 2]};
+}
+__device__ inline void stp(double *a, int s, v3<double> v) {
+    double2 *q = reinterpret_cast<double2 *>(a + 4 * (size_t)s);
+    q[0] = make_double2(v.x, v.y);
+    q[1] = make_double2(v.z, 0.0);
+}
+__device__ inline void st3(double *a, int n, int s, v3<double> v) {
+    a[s] = v.x;
+    a[n + s] = v.y;
+    a[2 * n + s] = v.z;
+}
+__device__ inline int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// Parameters of the force kernel in its precision T, and the term mask.
+template <typename T> struct TermTab {
+    TermParams<T> P;
+    uint32_t mask;
+};
+
+// Lanes per bead of the force kernel, and its launch (forces.hip, a
+// translation unit of its own compiled with multiply-add contraction).
+constexpr int FORCE_LPB = 8;
+void launch_force(bool mixed, bool eval, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
+                  const TermTab<float> &Tf, const TermTab<double> &Td, int p);
+
+}  // namespace xmd_k

@@ -263,17 +263,14 @@ struct xmd_engine {
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2); });
         launch("k_fp_write", [&] { k_fp_write<<<nb, BLK, 0, stream>>>(D, p); });
     }
-    static constexpr int LPB = 8;  // lanes per bead in k_force
-    template <typename T> void force_launch(const TermTab<T> &TT, int p) {
-        launch("k_force", [&] { k_force<T, false, LPB><<<nslot * LPB / BLK, BLK, 0, stream>>>(D, G, TT, p); });
-    }
     void step_seq(int p) {
         if (G.qa_on) {
             launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, 0, stream>>>(D, G, p, nfp_chunks, nct_chunks, 1); });
             launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
         }
-        if (P.precision == XMD_PREC_MIXED) force_launch(TTf, p);
-        else force_launch(TTd, p);
+        launch("k_force", [&] {
+            launch_force(P.precision == XMD_PREC_MIXED, false, nslot, stream, D, G, TTf, TTd, p);
+        });
         launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
         launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1); });
         rebuild_seq(D.pos[1 - p], p);
@@ -391,6 +388,7 @@ void derive(xmd_engine &E, const xmd_topology *t) {
     Q.bb_rmin = bb_rmin;
     Q.bs_depth = p.lj_bs_depth;
     Q.bs_rmin = bs_rmin;
+    Q.req_max = mfmd;
 
     auto fill = [&](auto &TT) {
         using T = std::remove_reference_t<decltype(TT.P.H1)>;
@@ -570,10 +568,10 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
         if (t->nat_chir_factor) cf[s] = t->nat_chir_factor[g];
     }
     // positions and state
-    std::vector<double> pos(3 * n, 0.0);
+    std::vector<double> pos(4 * (size_t)n, 0.0);  // AoS (x, y, z, 0) per slot
     for (int g = 0; g < E.nbeads; ++g) {
         int s = E.slot_of_bead[g];
-        for (int d = 0; d < 3; ++d) pos[d * n + s] = t->r0[d * E.nbeads + g];
+        for (int d = 0; d < 3; ++d) pos[4 * (size_t)s + d] = t->r0[d * E.nbeads + g];
     }
     // QA sync (D7)
     std::vector<uint32_t> sync(n, 0);
@@ -598,6 +596,7 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     rmd[3] = P.lj_bs_rmin;
     for (int a1 = 0; a1 < XMD_NUM_AA; ++a1)
         for (int a2 = 0; a2 < XMD_NUM_AA; ++a2) rmd[4 + a1 + a2 * XMD_NUM_AA] = ss_rmax[a1 * XMD_NUM_AA + a2];
+    for (double r : rmd) E.G.qa.req_max = std::max(E.G.qa.req_max, r);
     std::vector<double> charge(P.aa_charge, P.aa_charge + XMD_NUM_AA);
     std::vector<int8_t> ptype(P.aa_polarization, P.aa_polarization + XMD_NUM_AA);
     Mat J = step_matrix();
@@ -649,7 +648,7 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     D.pos[0] = E.upload(pos);
     D.pos[1] = E.upload(pos);
     D.y = E.alloc<double>(15 * (size_t)n);
-    D.orig_r = E.alloc<double>(3 * (size_t)n);
+    D.orig_r = E.alloc<double>(4 * (size_t)n);
     D.Fbuf = E.alloc<double>(3 * (size_t)n);
     D.pid_side = E.alloc<double>(6 * (size_t)n);
     D.Vbead = E.alloc<double>(n);
@@ -842,16 +841,32 @@ void xmd_engine_destroy(xmd_engine *e) { delete e; }
 
 void *xmd_stream(xmd_engine *e) { return e ? (void *)e->stream : nullptr; }
 
+// Positions of slots [s0, s0 + nb) from / to host SoA (x | y | z); the
+// device keeps them AoS, 4 doubles per slot.
+static void put_positions(xmd_engine *e, int s0, int nb, const double *r) {
+    std::vector<double> a(4 * (size_t)nb, 0.0);
+    for (int i = 0; i < nb; ++i)
+        for (int d = 0; d < 3; ++d) a[4 * (size_t)i + d] = r[(size_t)d * nb + i];
+    HIPCHK(hipMemcpy(e->D.pos[e->cur] + 4 * (size_t)s0, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+static void get_positions(xmd_engine *e, int s0, int nb, double *r) {
+    std::vector<double> a(4 * (size_t)nb);
+    HIPCHK(hipMemcpy(a.data(), e->D.pos[e->cur] + 4 * (size_t)s0, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; ++i)
+        for (int d = 0; d < 3; ++d) r[(size_t)d * nb + i] = a[4 * (size_t)i + d];
+}
+
 xmd_status xmd_set_state(xmd_engine *e, int32_t rep, const double *y, uint64_t rng, double t) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep || !y) throw Fail{XMD_ERR_ARG, "bad replica or null state"};
         int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep], n = e->nslot;
         HIPCHK(hipStreamSynchronize(e->stream));
-        for (int k = 0; k < 6; ++k)
+        put_positions(e, s0, nb, y);
+        for (int k = 1; k < 6; ++k)
             for (int d = 0; d < 3; ++d) {
                 const double *src = y + (3 * k + d) * nb;
-                double *dst = k == 0 ? e->D.pos[e->cur] + (size_t)d * n + s0 : e->D.y + (size_t)(3 * (k - 1) + d) * n + s0;
+                double *dst = e->D.y + (size_t)(3 * (k - 1) + d) * n + s0;
                 HIPCHK(hipMemcpy(dst, src, nb * sizeof(double), hipMemcpyHostToDevice));
             }
         HIPCHK(hipMemcpy(e->D.rng[e->cur] + rep, &rng, sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -867,10 +882,10 @@ xmd_status xmd_get_state(xmd_engine *e, int32_t rep, double *y, double *v, uint6
         int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep], n = e->nslot;
         HIPCHK(hipStreamSynchronize(e->stream));
         std::vector<double> y1(3 * (size_t)nb);
-        for (int k = 0; k < 6; ++k)
+        if (y) get_positions(e, s0, nb, y);
+        for (int k = 1; k < 6; ++k)
             for (int d = 0; d < 3; ++d) {
-                const double *src = k == 0 ? e->D.pos[e->cur] + (size_t)d * n + s0
-                                           : e->D.y + (size_t)(3 * (k - 1) + d) * n + s0;
+                const double *src = e->D.y + (size_t)(3 * (k - 1) + d) * n + s0;
                 double *dst = y ? y + (3 * k + d) * nb : nullptr;
                 if (k == 1) {
                     HIPCHK(hipMemcpy(y1.data() + d * nb, src, nb * sizeof(double), hipMemcpyDeviceToHost));
@@ -890,11 +905,9 @@ xmd_status xmd_set_positions(xmd_engine *e, int32_t rep, const double *r) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep || !r) throw Fail{XMD_ERR_ARG, "bad replica or null positions"};
-        int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep], n = e->nslot;
+        int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep];
         HIPCHK(hipStreamSynchronize(e->stream));
-        for (int d = 0; d < 3; ++d)
-            HIPCHK(hipMemcpy(e->D.pos[e->cur] + (size_t)d * n + s0, r + d * nb, nb * sizeof(double),
-                             hipMemcpyHostToDevice));
+        put_positions(e, s0, nb, r);
         e->pending_rb[rep] = 1;
     });
 }
@@ -980,15 +993,11 @@ xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double 
         if (e->G.qa_on && (m & (1u << T_QA)))
             k_qa_pairs<<<e->qa_grid, BLK, 0, e->stream>>>(e->D, e->G, p, e->nfp_chunks, e->nct_chunks, 0);
         int nb = e->nblk();
-        if (e->P.precision == XMD_PREC_MIXED) {
-            TermTab<float> TT = e->TTf;
-            TT.mask = m;
-            k_force<float, true, xmd_engine::LPB><<<e->nslot * xmd_engine::LPB / BLK, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
-        } else {
-            TermTab<double> TT = e->TTd;
-            TT.mask = m;
-            k_force<double, true, xmd_engine::LPB><<<e->nslot * xmd_engine::LPB / BLK, BLK, 0, e->stream>>>(e->D, e->G, TT, p);
-        }
+        TermTab<float> Tf = e->TTf;
+        TermTab<double> Td = e->TTd;
+        Tf.mask = m;
+        Td.mask = m;
+        launch_force(e->P.precision == XMD_PREC_MIXED, true, e->nslot, e->stream, e->D, e->G, Tf, Td, p);
         k_gather_force<<<nb, BLK, 0, e->stream>>>(e->D);
         HIPCHK(hipGetLastError());
         int n = e->nslot;

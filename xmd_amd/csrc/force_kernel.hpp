@@ -1,0 +1,234 @@
+// force_kernel.hpp -- the per-bead force kernel (included by forces.hip
+// only, which is compiled with -ffp-contract=fast: the force arithmetic may
+// fuse multiply-adds; list, QA and noise code elsewhere keeps exact
+// evaluation order).
+#pragma once
+#include "common.hpp"
+
+namespace xmd_k {
+
+// Force on every slot from every enabled term.  LPB consecutive lanes share
+// a bead: the 8 bonded tasks (4 quads, 3 angles, the two tethers) and the
+// neighbour-row entries are strided over the lanes, then a shuffle tree
+// (fixed order, hence deterministic) sums the lanes' partial forces.  The
+// result goes to Fbuf (plus the PID spill onto chain neighbours); the
+// integrator runs in k_integrate.
+template <typename T>
+__device__ inline void add3(double &x, double &y, double &z, v3<T> f) {
+    x += (double)f.x;
+    y += (double)f.y;
+    z += (double)f.z;
+}
+
+// Element k (runtime) of a small local array by selects on register values.
+// The empty asm statements pin each element in registers first; otherwise
+// the selects fold into an indexed load and the array moves to scratch.
+template <typename T> __device__ inline v3<T> in_regs(v3<T> v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
+    return v;
+}
+template <typename T> __device__ inline v3<T> pick3(const v3<T> a[3], int k) {
+    const v3<T> a0 = in_regs(a[0]), a1 = in_regs(a[1]), a2 = in_regs(a[2]);
+    return k == 0 ? a0 : (k == 1 ? a1 : a2);
+}
+template <typename T> __device__ inline v3<T> pick4(const v3<T> a[4], int k) {
+    const v3<T> a0 = in_regs(a[0]), a1 = in_regs(a[1]), a2 = in_regs(a[2]), a3 = in_regs(a[3]);
+    return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+}
+template <int LPB> __device__ inline double lane_sum(double v) {
+    for (int o = LPB / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <typename T, bool EVAL, int LPB>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
+    const int n = D.nslot;
+    const int t = blockIdx.x * BLK + threadIdx.x;
+    const int s = t / LPB, sub = t % LPB;
+    const int rep = D.slot_rep[s];
+    const uint8_t bf = D.bflags[s];
+    const bool active = bf & B_ACTIVE;
+    const double *Pp = D.pos[p];
+    const TermParams<T> &P = TT.P;
+    const uint32_t mask = TT.mask;
+    double Fx = 0, Fy = 0, Fz = 0, V = 0;
+    double Vt[EVAL ? NUM_TERMS : 1];
+    for (int k = 0; k < (EVAL ? NUM_TERMS : 1); ++k) Vt[k] = 0;
+    auto addV = [&](int term, double v) {
+        V += v;
+        if constexpr (EVAL) Vt[term] += v;
+    };
+    double tpx = 0, tpy = 0, tpz = 0, tnx = 0, tny = 0, tnz = 0;  // PID spill to s-1, s+1
+    if (active) {
+        const v3<double> ri = ldp(Pp, s);
+        const Box box = D.box[rep];
+        const int ch = D.chain[s];
+        auto in_chain = [&](int a) { return a >= 0 && a < n && D.chain[a] == ch; };
+        // bonded tasks: 0..3 quad starting at s-3+task, 4..6 angle starting
+        // at s-2+(task-4), 7 tethers
+        for (int task = sub; task < 8; task += LPB) {
+            if (task < 4) {
+                int a = task, first = s - 3 + a;
+                if (!(mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND)))) continue;
+                if (!in_chain(first) || !in_chain(first + 3)) continue;
+                v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2),
+                           q3 = ldp(Pp, first + 3);
+                v3<T> r12 = cvt<T>(q1 - q0), r23 = cvt<T>(q2 - q1), r34 = cvt<T>(q3 - q2);
+                v3<T> out[4];
+                if (mask & (1u << T_CHIRAL)) {
+                    double nc = D.chir_nat[first];
+                    if (nc == nc) {
+                        T v = chiral<T>(r12, r23, r34, P.e_chi, (T)nc, (T)D.chir_fac[first], out);
+                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
+                        if (a == 3) addV(T_CHIRAL, (double)v);
+                    }
+                }
+                double ph = D.phi_nat[first];
+                if (ph == ph) {
+                    if (mask & (1u << T_CND)) {
+                        T v = nat_dihedral<T>(r12, r23, r34, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
+                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
+                        if (a == 3) addV(T_CND, (double)v);
+                    }
+                    if (mask & (1u << T_SND)) {
+                        T v = nat_dihedral<T>(r12, r23, r34, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
+                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
+                        if (a == 3) addV(T_SND, (double)v);
+                    }
+                }
+            } else if (task < 7) {
+                int a = task - 4, first = s - 2 + a;
+                if (!(mask & (1u << T_NAT_ANG))) continue;
+                if (!in_chain(first) || !in_chain(first + 2)) continue;
+                double th = D.theta_nat[first];
+                if (th != th) continue;
+                v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
+                v3<T> out[3];
+                T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
+                add3(Fx, Fy, Fz, pick3(out, 2 - a));
+                if (a == 2) addV(T_NAT_ANG, (double)v);
+            } else {
+                // FP64 in both forms: stiff bonds cancel in r - r_nat
+                if (!(mask & (1u << T_TETHER))) continue;
+                v3<double> out[2];
+                if (bf & B_PREV) {
+                    tether<double>(ri - ldp(Pp, s - 1), G.H1, G.H2, D.tether_nat[s - 1], out);
+                    add3(Fx, Fy, Fz, out[1]);
+                }
+                if (bf & B_NEXT) {
+                    double v = tether<double>(ldp(Pp, s + 1) - ri, G.H1, G.H2, D.tether_nat[s], out);
+                    add3(Fx, Fy, Fz, out[0]);
+                    addV(T_TETHER, v);
+                }
+            }
+        }
+        // native contacts (Go)
+        if (mask & (1u << T_GO)) {
+            for (int k = D.nat_start[s] + sub; k < D.nat_start[s + 1]; k += LPB) {
+                if (!D.nat_act[k]) continue;
+                int j = D.nat_j[k];
+                v3<double> d = wrap(box, ldp(Pp, j) - ri);
+                FV<T> f = go_pair<T>(cvt<T>(d), P.go_depth, (T)D.nat_d[k]);
+                add3(Fx, Fy, Fz, f.F);
+                if (s < j) addV(T_GO, (double)f.V);
+            }
+        }
+        // neighbour row: Pauli, Debye-Hueckel, PID
+        uint32_t rmask = mask & ((1u << T_PAULI) | (1u << T_CONST_ES) | (1u << T_REL_ES) | (1u << T_PID));
+        if (rmask && !(D.err[0] & E_NL_CAP)) {
+            const int len = D.nl_len[s];
+            const int base = D.slice_start[s >> 6] + (s & 63);
+            const int es_term = P.es_relative ? T_REL_ES : T_CONST_ES;
+            const bool es_on = mask & (1u << es_term);
+            const T q_s = (T)D.charge[D.atype[s]];
+            const v3<double> rsp = (bf & B_PREV) ? ldp(Pp, s - 1) : ri;
+            const v3<double> rsn = (bf & B_NEXT) ? ldp(Pp, s + 1) : ri;
+            for (int k = sub; k < len; k += LPB) {
+                const int e = base + 64 * k;
+                const uint8_t fl = D.nl_flag[e];
+                if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
+                const int j = D.nl_j[e];
+                const v3<double> rj = ldp(Pp, j);
+                const v3<double> dd = wrap(box, rj - ri);
+                const v3<T> d = cvt<T>(dd);
+                const bool owner = s < j;
+                if ((fl & F_PAULI) && (mask & (1u << T_PAULI))) {
+                    FV<T> f = pauli_pair<T>(d, P.pauli_depth, P.pauli_r_excl);
+                    add3(Fx, Fy, Fz, f.F);
+                    if (owner) addV(T_PAULI, (double)f.V);
+                }
+                if ((fl & F_ES) && es_on) {
+                    T q1q2 = owner ? q_s * (T)D.charge[D.atype[j]] : (T)D.charge[D.atype[j]] * q_s;
+                    FV<T> f = es_pair<T>(d, q1q2, P.es_Vf, P.es_sdi, P.es_relative);
+                    add3(Fx, Fy, Fz, f.F);
+                    if (owner) addV(es_term, (double)f.V);
+                }
+                if ((fl & F_PID) && (mask & (1u << T_PID))) {
+                    // The bundle from this bead's side: s in the first slot.
+                    // lambda(psi_1) lambda(psi_2) LJ(r) is symmetric in the
+                    // two sides, so out[0..2] are the reference's forces on
+                    // (s-1, s, s+1) whichever bead of the pair is i1; the
+                    // energy is counted by the owner (s = i1).
+                    const v3<double> rjp = ldp(Pp, j - 1), rjn = ldp(Pp, j + 1);
+                    const int i1 = owner ? s : j, i2 = owner ? j : s;
+                    const int ty = 20 * D.atype[i1] + D.atype[i2];
+                    v3<T> out[6];
+                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], ri - rsp, rsn - ri,
+                                        rsn - rsp, dd, rj - rjp, rjn - rj, rjn - rjp, -dd, dd, out);
+                    add3(Fx, Fy, Fz, out[1]);
+                    add3(tpx, tpy, tpz, out[0]);
+                    add3(tnx, tny, tnz, out[2]);
+                    if (owner) addV(T_PID, (double)v);
+                }
+            }
+        }
+        // quasi-adiabatic contacts through the bead's contact slots
+        if (mask & (1u << T_QA)) {
+            const int q = D.qn[s], tot = D.ct_cap[D.nrep];
+            for (int k = sub; k < q; k += LPB) {
+                int ci = D.qslot[k * n + s];
+                double fx = D.ct_F[ci], fy = D.ct_F[tot + ci], fz = D.ct_F[2 * tot + ci];
+                if (D.ct_i1[ci] == s) {
+                    Fx += fx;
+                    Fy += fy;
+                    Fz += fz;
+                    addV(T_QA, D.ct_V[ci]);
+                } else {
+                    Fx -= fx;
+                    Fy -= fy;
+                    Fz -= fz;
+                }
+            }
+        }
+    }
+    Fx = lane_sum<LPB>(Fx);
+    Fy = lane_sum<LPB>(Fy);
+    Fz = lane_sum<LPB>(Fz);
+    V = lane_sum<LPB>(V);
+    tpx = lane_sum<LPB>(tpx);
+    tpy = lane_sum<LPB>(tpy);
+    tpz = lane_sum<LPB>(tpz);
+    tnx = lane_sum<LPB>(tnx);
+    tny = lane_sum<LPB>(tny);
+    tnz = lane_sum<LPB>(tnz);
+    if constexpr (EVAL)
+        for (int k = 0; k < NUM_TERMS; ++k) Vt[k] = lane_sum<LPB>(Vt[k]);
+    if (active && sub == 0) {
+        if constexpr (EVAL)
+            for (int k = 0; k < NUM_TERMS; ++k) D.Vterm[k * n + s] = Vt[k];
+        D.Fbuf[s] = Fx;
+        D.Fbuf[n + s] = Fy;
+        D.Fbuf[2 * n + s] = Fz;
+        D.Vbead[s] = V;
+        double *ps = D.pid_side;
+        ps[s] = tpx;
+        ps[n + s] = tpy;
+        ps[2 * n + s] = tpz;
+        ps[3 * n + s] = tnx;
+        ps[4 * n + s] = tny;
+        ps[5 * n + s] = tnz;
+    }
+}
+
+}  // namespace xmd_k

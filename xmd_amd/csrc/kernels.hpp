@@ -15,94 +15,9 @@
 //   k_finalize    RNG/time advance, verify decision       (nl/verify.cpp)
 //   k_cell_* / k_nl_* / k_fp_write  Verlet rebuild        (nl/legacy_update.cpp + update_X)
 #pragma once
-#include <hip/hip_runtime.h>
-
-#include "xmd_dev.hpp"
+#include "common.hpp"
 
 namespace xmd_k {
-using namespace xmd_dev;
-
-constexpr int WAVE = 64;
-constexpr int BLK = 256;
-constexpr int MAXQ = 16;
-constexpr int NUM_TERMS = 11;
-
-enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16 };
-enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
-enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
-enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID };
-
-// Derived parameters in double (list and QA decisions, integrator).
-struct Derived {
-    double dt, dt_inv, gamma, half_dt2;
-    double H1, H2;  // tether constants (FP64 in both forms)
-    double pad, req_r;
-    double pauli_min_inv, es_min_inv, pid_min_inv, qa_min_inv;
-    int pauli_on, es_on, pid_on, qa_on, go_on;
-    int lang_on;
-    QaParams qa;
-};
-
-// Every device pointer of an engine (passed by value to kernels).
-struct Dev {
-    int nslot, nrep;
-    // replicas
-    const int *rep_slot_off, *rep_n;
-    const Box *box;
-    const double *noise_sd;
-    uint64_t *rng[2];
-    double *tt[2];
-    unsigned long long *maxd[2];
-    int *rb_now, *force_rb, *any_rb, *err;
-    long long *n_rebuild;
-    const int *cell_off, *cell_dim;  // [nrep+1], [3*nrep]
-    int *cell_cnt, *cell_start, *cell_end, *cell_cursor, *cell_beads, *bead_cell;
-    // slots
-    const int *slot_rep, *chain;
-    const uint8_t *bflags;
-    const int8_t *atype;
-    double *pos[2];
-    double *y;  // y1..y5, 15*nslot
-    double *orig_r, *Fbuf, *pid_side, *Vbead, *Vterm;
-    const double *tether_nat, *theta_nat, *phi_nat, *chir_nat, *chir_fac;
-    // native contacts (rows of slot s: nat_start[s]..nat_start[s+1])
-    const int *nat_start, *nat_j;
-    const double *nat_d;
-    uint8_t *nat_act;
-    // neighbour rows
-    const int *nl_base, *nl_cap;
-    int *slice_start, *slice_w, *nl_len, *nl_half, *nl_j;
-    uint8_t *nl_flag;
-    // QA free pairs, candidates
-    const int *fp_base, *fp_cap, *fp_chunk_rep;
-    int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt, *fp_slice_start;
-    uint8_t *fp_vac;
-    int *cand_cnt, *cand_fp, *cand_type;
-    uint32_t *cand_d1, *cand_d2;
-    // QA contacts, removals
-    const int *ct_base, *ct_cap, *ct_chunk_rep;
-    int *ct_count, *ct_i1, *ct_i2, *ct_type, *ct_status;
-    double *ct_ref, *ct_F, *ct_V;
-    uint32_t *ct_d1, *ct_d2;
-    uint8_t *ct_vac;
-    int *rm_cnt, *rm_idx;
-    uint32_t *sync;
-    int *qslot, *qn;
-    // tables
-    const double *ss_depth, *ss_rmin, *ss_rmax, *req_min_dist, *charge;
-    const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
-    const int8_t *ptype;
-    const uint8_t *bm_fix;
-    const uint64_t *jrow, *jlane, *jadv;  // jadv: rows of J^(4N) per replica
-};
-
-__device__ inline v3<double> ld3(const double *a, int n, int s) { return {a[s], a[n + s], a[2 * n + s]}; }
-__device__ inline void st3(double *a, int n, int s, v3<double> v) {
-    a[s] = v.x;
-    a[n + s] = v.y;
-    a[2 * n + s] = v.z;
-}
-__device__ inline int lane_id() { return threadIdx.x & (WAVE - 1); }
 
 // ------------------------------------------------------------ RNG jumps ---
 // y = M x over GF(2), M given by 64 rows (lane r holds row r): every lane
@@ -153,21 +68,26 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
         if (sift) {
             if (local < count && !D.fp_vac[idx]) {
                 int i1 = D.fp_i1[idx], i2 = D.fp_i2[idx];
-                uint8_t f1 = D.bflags[i1], f2 = D.bflags[i2];
-                bool ok1 = (f1 & B_PREV) && (f1 & B_NEXT), ok2 = (f2 & B_PREV) && (f2 & B_NEXT);
-                v3<double> n1, h1, n2, h2;
-                v3<double> r1 = ld3(P, n, i1), r2 = ld3(P, n, i2);
-                qa_nh(ok1, ok1 ? ld3(P, n, i1 - 1) : r1, r1, ok1 ? ld3(P, n, i1 + 1) : r1, n1, h1);
-                qa_nh(ok2, ok2 ? ld3(P, n, i2 - 1) : r2, r2, ok2 ? ld3(P, n, i2 + 1) : r2, n2, h2);
+                v3<double> r1 = ldp(P, i1), r2 = ldp(P, i2);
                 v3<double> r12 = r_uv(box, r1, r2);
-                type = qa_sift(G.qa, D.req_min_dist, D.ptype, r12, n1, h1, n2, h2, D.atype[i1], D.atype[i2],
-                               D.sync[i1], D.sync[i2], d1, d2);
-                flag = type != 0;
+                // qa_sift rejects a pair of type k when 1 > r12_rn req[k]; with
+                // req[k] <= req_max that holds for every k once it holds for
+                // req_max (rounding is monotone), so far pairs skip n and h
+                if (!(1.0 > norm_inv(r12) * G.qa.req_max)) {
+                    uint8_t f1 = D.bflags[i1], f2 = D.bflags[i2];
+                    bool ok1 = (f1 & B_PREV) && (f1 & B_NEXT), ok2 = (f2 & B_PREV) && (f2 & B_NEXT);
+                    v3<double> n1, h1, n2, h2;
+                    qa_nh(ok1, ok1 ? ldp(P, i1 - 1) : r1, r1, ok1 ? ldp(P, i1 + 1) : r1, n1, h1);
+                    qa_nh(ok2, ok2 ? ldp(P, i2 - 1) : r2, r2, ok2 ? ldp(P, i2 + 1) : r2, n2, h2);
+                    type = qa_sift(G.qa, D.req_min_dist, D.ptype, r12, n1, h1, n2, h2, D.atype[i1], D.atype[i2],
+                                   D.sync[i1], D.sync[i2], d1, d2);
+                    flag = type != 0;
+                }
             }
         } else {
             if (local < count && !D.ct_vac[idx]) {
                 int i1 = D.ct_i1[idx], i2 = D.ct_i2[idx];
-                v3<double> r12 = r_uv(box, ld3(P, n, i1), ld3(P, n, i2));
+                v3<double> r12 = r_uv(box, ldp(P, i1), ldp(P, i2));
                 double depth, rmin, rmax;
                 int ctype = D.ct_type[idx], status = D.ct_status[idx], next;
                 qa_lj(G.qa, D.ss_depth, D.ss_rmin, D.ss_rmax, ctype, depth, rmin, rmax);
@@ -431,7 +351,7 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
     double disp = 0.0;
     if (active) {
         double *y = D.y;
-        v3<double> y0 = ld3(D.pos[p], n, s);
+        v3<double> y0 = ldp(D.pos[p], s);
         v3<double> y1 = ld3(y, n, s), y2 = ld3(y + 3 * n, n, s), y3 = ld3(y + 6 * n, n, s);
         v3<double> y4 = ld3(y + 9 * n, n, s), y5 = ld3(y + 12 * n, n, s);
         if (G.lang_on) {
@@ -454,13 +374,13 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
             y3 = y3 + (4.0 * y4 + 10.0 * y5);
             y4 = y4 + 5.0 * y5;
         }
-        st3(D.pos[1 - p], n, s, y0);
+        stp(D.pos[1 - p], s, y0);
         st3(y, n, s, y1);
         st3(y + 3 * n, n, s, y2);
         st3(y + 6 * n, n, s, y3);
         st3(y + 9 * n, n, s, y4);
         st3(y + 12 * n, n, s, y5);
-        v3<double> dr = r_uv(D.box[rep], y0, ld3(D.orig_r, n, s));
+        v3<double> dr = r_uv(D.box[rep], y0, ldp(D.orig_r, s));
         disp = norm(dr);
         if (!(disp == disp) || !(fabs(y0.x) < 1e300)) disp = __builtin_nan("");
     }
@@ -474,234 +394,6 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
 }
 
 // ------------------------------------------------------------- forces ---
-// Force on every slot from every enabled term.  LPB consecutive lanes share
-// a bead: the 8 bonded tasks (4 quads, 3 angles, the two tethers) and the
-// neighbour-row entries are strided over the lanes, then a shuffle tree
-// (fixed order, hence deterministic) sums the lanes' partial forces.  The
-// result goes to Fbuf (plus the PID spill onto chain neighbours); the
-// integrator runs in k_integrate.
-template <typename T> struct TermTab {
-    TermParams<T> P;
-    uint32_t mask;
-};
-
-template <typename T>
-__device__ inline void add3(double &x, double &y, double &z, v3<T> f) {
-    x += (double)f.x;
-    y += (double)f.y;
-    z += (double)f.z;
-}
-
-// Element k (runtime) of a small local array by selects on register values.
-// The empty asm statements pin each element in registers first; otherwise
-// the selects fold into an indexed load and the array moves to scratch.
-template <typename T> __device__ inline v3<T> in_regs(v3<T> v) {
-    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
-    return v;
-}
-template <typename T> __device__ inline v3<T> pick3(const v3<T> a[3], int k) {
-    const v3<T> a0 = in_regs(a[0]), a1 = in_regs(a[1]), a2 = in_regs(a[2]);
-    return k == 0 ? a0 : (k == 1 ? a1 : a2);
-}
-template <typename T> __device__ inline v3<T> pick4(const v3<T> a[4], int k) {
-    const v3<T> a0 = in_regs(a[0]), a1 = in_regs(a[1]), a2 = in_regs(a[2]), a3 = in_regs(a[3]);
-    return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
-}
-template <int LPB> __device__ inline double lane_sum(double v) {
-    for (int o = LPB / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-template <typename T, bool EVAL, int LPB>
-__global__ __launch_bounds__(BLK) void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
-    const int n = D.nslot;
-    const int t = blockIdx.x * BLK + threadIdx.x;
-    const int s = t / LPB, sub = t % LPB;
-    const int rep = D.slot_rep[s];
-    const uint8_t bf = D.bflags[s];
-    const bool active = bf & B_ACTIVE;
-    const double *Pp = D.pos[p];
-    const TermParams<T> &P = TT.P;
-    const uint32_t mask = TT.mask;
-    double Fx = 0, Fy = 0, Fz = 0, V = 0;
-    double Vt[EVAL ? NUM_TERMS : 1];
-    for (int k = 0; k < (EVAL ? NUM_TERMS : 1); ++k) Vt[k] = 0;
-    auto addV = [&](int term, double v) {
-        V += v;
-        if constexpr (EVAL) Vt[term] += v;
-    };
-    double tpx = 0, tpy = 0, tpz = 0, tnx = 0, tny = 0, tnz = 0;  // PID spill to s-1, s+1
-    if (active) {
-        const v3<double> ri = ld3(Pp, n, s);
-        const Box box = D.box[rep];
-        const int ch = D.chain[s];
-        auto in_chain = [&](int a) { return a >= 0 && a < n && D.chain[a] == ch; };
-        // bonded tasks: 0..3 quad starting at s-3+task, 4..6 angle starting
-        // at s-2+(task-4), 7 tethers
-        for (int task = sub; task < 8; task += LPB) {
-            if (task < 4) {
-                int a = task, first = s - 3 + a;
-                if (!(mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND)))) continue;
-                if (!in_chain(first) || !in_chain(first + 3)) continue;
-                v3<double> q0 = ld3(Pp, n, first), q1 = ld3(Pp, n, first + 1), q2 = ld3(Pp, n, first + 2),
-                           q3 = ld3(Pp, n, first + 3);
-                v3<T> r12 = cvt<T>(q1 - q0), r23 = cvt<T>(q2 - q1), r34 = cvt<T>(q3 - q2);
-                v3<T> out[4];
-                if (mask & (1u << T_CHIRAL)) {
-                    double nc = D.chir_nat[first];
-                    if (nc == nc) {
-                        T v = chiral<T>(r12, r23, r34, P.e_chi, (T)nc, (T)D.chir_fac[first], out);
-                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                        if (a == 3) addV(T_CHIRAL, (double)v);
-                    }
-                }
-                double ph = D.phi_nat[first];
-                if (ph == ph) {
-                    if (mask & (1u << T_CND)) {
-                        T v = nat_dihedral<T>(r12, r23, r34, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
-                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                        if (a == 3) addV(T_CND, (double)v);
-                    }
-                    if (mask & (1u << T_SND)) {
-                        T v = nat_dihedral<T>(r12, r23, r34, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
-                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                        if (a == 3) addV(T_SND, (double)v);
-                    }
-                }
-            } else if (task < 7) {
-                int a = task - 4, first = s - 2 + a;
-                if (!(mask & (1u << T_NAT_ANG))) continue;
-                if (!in_chain(first) || !in_chain(first + 2)) continue;
-                double th = D.theta_nat[first];
-                if (th != th) continue;
-                v3<double> q0 = ld3(Pp, n, first), q1 = ld3(Pp, n, first + 1), q2 = ld3(Pp, n, first + 2);
-                v3<T> out[3];
-                T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
-                add3(Fx, Fy, Fz, pick3(out, 2 - a));
-                if (a == 2) addV(T_NAT_ANG, (double)v);
-            } else {
-                // FP64 in both forms: stiff bonds cancel in r - r_nat
-                if (!(mask & (1u << T_TETHER))) continue;
-                v3<double> out[2];
-                if (bf & B_PREV) {
-                    tether<double>(ri - ld3(Pp, n, s - 1), G.H1, G.H2, D.tether_nat[s - 1], out);
-                    add3(Fx, Fy, Fz, out[1]);
-                }
-                if (bf & B_NEXT) {
-                    double v = tether<double>(ld3(Pp, n, s + 1) - ri, G.H1, G.H2, D.tether_nat[s], out);
-                    add3(Fx, Fy, Fz, out[0]);
-                    addV(T_TETHER, v);
-                }
-            }
-        }
-        // native contacts (Go)
-        if (mask & (1u << T_GO)) {
-            for (int k = D.nat_start[s] + sub; k < D.nat_start[s + 1]; k += LPB) {
-                if (!D.nat_act[k]) continue;
-                int j = D.nat_j[k];
-                v3<double> d = wrap(box, ld3(Pp, n, j) - ri);
-                FV<T> f = go_pair<T>(cvt<T>(d), P.go_depth, (T)D.nat_d[k]);
-                add3(Fx, Fy, Fz, f.F);
-                if (s < j) addV(T_GO, (double)f.V);
-            }
-        }
-        // neighbour row: Pauli, Debye-Hueckel, PID
-        uint32_t rmask = mask & ((1u << T_PAULI) | (1u << T_CONST_ES) | (1u << T_REL_ES) | (1u << T_PID));
-        if (rmask && !(D.err[0] & E_NL_CAP)) {
-            const int len = D.nl_len[s];
-            const int base = D.slice_start[s >> 6] + (s & 63);
-            const int es_term = P.es_relative ? T_REL_ES : T_CONST_ES;
-            const bool es_on = mask & (1u << es_term);
-            const T q_s = (T)D.charge[D.atype[s]];
-            const v3<double> rsp = (bf & B_PREV) ? ld3(Pp, n, s - 1) : ri;
-            const v3<double> rsn = (bf & B_NEXT) ? ld3(Pp, n, s + 1) : ri;
-            for (int k = sub; k < len; k += LPB) {
-                const int e = base + 64 * k;
-                const uint8_t fl = D.nl_flag[e];
-                if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
-                const int j = D.nl_j[e];
-                const v3<double> rj = ld3(Pp, n, j);
-                const v3<double> dd = wrap(box, rj - ri);
-                const v3<T> d = cvt<T>(dd);
-                const bool owner = s < j;
-                if ((fl & F_PAULI) && (mask & (1u << T_PAULI))) {
-                    FV<T> f = pauli_pair<T>(d, P.pauli_depth, P.pauli_r_excl);
-                    add3(Fx, Fy, Fz, f.F);
-                    if (owner) addV(T_PAULI, (double)f.V);
-                }
-                if ((fl & F_ES) && es_on) {
-                    T q1q2 = owner ? q_s * (T)D.charge[D.atype[j]] : (T)D.charge[D.atype[j]] * q_s;
-                    FV<T> f = es_pair<T>(d, q1q2, P.es_Vf, P.es_sdi, P.es_relative);
-                    add3(Fx, Fy, Fz, f.F);
-                    if (owner) addV(es_term, (double)f.V);
-                }
-                if ((fl & F_PID) && (mask & (1u << T_PID))) {
-                    // The bundle from this bead's side: s in the first slot.
-                    // lambda(psi_1) lambda(psi_2) LJ(r) is symmetric in the
-                    // two sides, so out[0..2] are the reference's forces on
-                    // (s-1, s, s+1) whichever bead of the pair is i1; the
-                    // energy is counted by the owner (s = i1).
-                    const v3<double> rjp = ld3(Pp, n, j - 1), rjn = ld3(Pp, n, j + 1);
-                    const int i1 = owner ? s : j, i2 = owner ? j : s;
-                    const int ty = 20 * D.atype[i1] + D.atype[i2];
-                    v3<T> out[6];
-                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], ri - rsp, rsn - ri,
-                                        rsn - rsp, dd, rj - rjp, rjn - rj, rjn - rjp, -dd, dd, out);
-                    add3(Fx, Fy, Fz, out[1]);
-                    add3(tpx, tpy, tpz, out[0]);
-                    add3(tnx, tny, tnz, out[2]);
-                    if (owner) addV(T_PID, (double)v);
-                }
-            }
-        }
-        // quasi-adiabatic contacts through the bead's contact slots
-        if (mask & (1u << T_QA)) {
-            const int q = D.qn[s], tot = D.ct_cap[D.nrep];
-            for (int k = sub; k < q; k += LPB) {
-                int ci = D.qslot[k * n + s];
-                double fx = D.ct_F[ci], fy = D.ct_F[tot + ci], fz = D.ct_F[2 * tot + ci];
-                if (D.ct_i1[ci] == s) {
-                    Fx += fx;
-                    Fy += fy;
-                    Fz += fz;
-                    addV(T_QA, D.ct_V[ci]);
-                } else {
-                    Fx -= fx;
-                    Fy -= fy;
-                    Fz -= fz;
-                }
-            }
-        }
-    }
-    Fx = lane_sum<LPB>(Fx);
-    Fy = lane_sum<LPB>(Fy);
-    Fz = lane_sum<LPB>(Fz);
-    V = lane_sum<LPB>(V);
-    tpx = lane_sum<LPB>(tpx);
-    tpy = lane_sum<LPB>(tpy);
-    tpz = lane_sum<LPB>(tpz);
-    tnx = lane_sum<LPB>(tnx);
-    tny = lane_sum<LPB>(tny);
-    tnz = lane_sum<LPB>(tnz);
-    if constexpr (EVAL)
-        for (int k = 0; k < NUM_TERMS; ++k) Vt[k] = lane_sum<LPB>(Vt[k]);
-    if (active && sub == 0) {
-        if constexpr (EVAL)
-            for (int k = 0; k < NUM_TERMS; ++k) D.Vterm[k * n + s] = Vt[k];
-        D.Fbuf[s] = Fx;
-        D.Fbuf[n + s] = Fy;
-        D.Fbuf[2 * n + s] = Fz;
-        D.Vbead[s] = V;
-        double *ps = D.pid_side;
-        ps[s] = tpx;
-        ps[n + s] = tpy;
-        ps[2 * n + s] = tpz;
-        ps[3 * n + s] = tnx;
-        ps[4 * n + s] = tny;
-        ps[5 * n + s] = tnz;
-    }
-}
-
 // Integration when PID spilled forces onto chain neighbours.
 __global__ __launch_bounds__(BLK) void k_integrate(Dev D, Derived G, int p) {
     const int n = D.nslot;
@@ -800,9 +492,8 @@ __global__ __launch_bounds__(BLK) void k_cell_count(Dev D, const double *P, int 
     if (D.cell_off[rep + 1] == c0) return;
     const int *nd = D.cell_dim + 3 * rep;
     Box b = D.box[rep];
-    int n = D.nslot;
-    int cx = cell_coord(P[s], b.cx, nd[0]), cy = cell_coord(P[n + s], b.cy, nd[1]),
-        cz = cell_coord(P[2 * n + s], b.cz, nd[2]);
+    const v3<double> r = ldp(P, s);
+    int cx = cell_coord(r.x, b.cx, nd[0]), cy = cell_coord(r.y, b.cy, nd[1]), cz = cell_coord(r.z, b.cz, nd[2]);
     int c = c0 + (cz * nd[1] + cy) * nd[0] + cx;
     D.bead_cell[s] = c;
     atomicAdd(D.cell_cnt + c, 1);
@@ -894,7 +585,7 @@ __global__ __launch_bounds__(BLK) void k_cell_fill(Dev D, int p) {
 __device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &box, int s, int j, v3<double> rs,
                                      const double *P) {
     if (D.chain[j] == D.chain[s] && (s > j ? s - j : j - s) < 3) return 0;
-    v3<double> d = wrap(box, ld3(P, D.nslot, j) - rs);
+    v3<double> d = wrap(box, ldp(P, j) - rs);
     double dist = norm(d);
     if (!(dist < G.req_r)) return 0;
     // native?  rows are short and sorted
@@ -967,7 +658,7 @@ __global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double
     int cnt = 0;
     if (D.bflags[s] & B_ACTIVE) {
         const Box box = D.box[rep];
-        const v3<double> rs = ld3(P, D.nslot, s);
+        const v3<double> rs = ldp(P, s);
         for_candidates_wave(D, s, rep, [&](int j) {
             if (j >= 0 && (pair_flags(D, G, box, s, j, rs, P) & (F_PAULI | F_ES | F_PID | F_QAFREE))) ++cnt;
         });
@@ -995,7 +686,7 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
     const bool active = D.bflags[s] & B_ACTIVE;
     const int n = D.nslot;
     const Box box = D.box[rep];
-    const v3<double> rs = ld3(P, n, s);
+    const v3<double> rs = ldp(P, s);
     const int base = D.slice_start[s >> 6] + (s & 63);
     const int len = active ? D.nl_len[s] : 0;
     const bool in_lds = len <= MAXSORT;
@@ -1052,12 +743,12 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
     if (l == 0) {
         D.nl_half[s] = below;
         if (nfp) atomicAdd(D.fp_slice_cnt + (s >> 6), nfp);
-        st3(D.orig_r, n, s, rs);
+        stp(D.orig_r, s, rs);
     }
     // Go activity (go.cpp:11-22)
     for (int k = D.nat_start[s] + l; k < D.nat_start[s + 1]; k += WAVE) {
         int j = D.nat_j[k];
-        v3<double> d = wrap(box, ld3(P, n, j) - rs);
+        v3<double> d = wrap(box, ldp(P, j) - rs);
         double cutoff = 2.0 * D.nat_d[k];
         D.nat_act[k] = norm(d) < cutoff + G.pad;
     }
@@ -1110,12 +801,12 @@ __global__ __launch_bounds__(BLK) void k_nl_export(Dev D, Derived G, int *cnt, c
     int rep = D.slot_rep[s];
     const int n = D.nslot;
     Box box = D.box[rep];
-    v3<double> rs = ld3(D.orig_r, n, s);
+    v3<double> rs = ldp(D.orig_r, s);
     int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
     int k = mode ? off[s] : 0;
     for (int j = s + 1; j < b; ++j) {
         if (D.chain[j] == D.chain[s] && j - s < 3) continue;
-        v3<double> d = wrap(box, ld3(D.orig_r, n, j) - rs);
+        v3<double> d = wrap(box, ldp(D.orig_r, j) - rs);
         if (norm(d) < G.req_r) {
             if (mode) {
                 bool native = false;

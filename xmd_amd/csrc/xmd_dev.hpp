@@ -446,6 +446,7 @@ struct QaParams {
     double min_cos_hr, min_cos_hh, max_cos_nr;
     double cycle_time, cycle_time_inv, factor;
     double bb_depth, bb_rmin, bs_depth, bs_rmin;
+    double req_max;  // largest required formation distance over contact types
 };
 // sync_data packed as 4 int8 in a uint32 (back, side_all, side_polar,
 // side_hydrophobic), xmd/include/xmd/forces/qa/sync_data.h:8-41.
