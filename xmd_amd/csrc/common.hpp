@@ -63,6 +63,7 @@ struct Dev {
     // QA free pairs, candidates
     const int *fp_base, *fp_cap, *fp_chunk_rep;
     int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt, *fp_slice_start;
+    int *fp_n;  // QA free pairs of each slot's row (set at rebuild)
     uint8_t *fp_vac;
     int *cand_cnt, *cand_fp, *cand_type;
     uint32_t *cand_d1, *cand_d2;

@@ -250,6 +250,10 @@ struct xmd_engine {
     }
 
     int nblk() const { return nslot / BLK; }
+    // one resident round of blocks of the grid-stride rebuild kernels (a
+    // partial second round would leave most of the chip idle on a full rebuild)
+    int grid_count = 0, grid_fill = 0;
+    int nl_grid(bool fill = false) const { return std::min(nslot / NL_WAVES, fill ? grid_fill : grid_count); }
 
     // ------------------------------------------------------------ steps ---
     void rebuild_seq(const double *Ppos, int p) {
@@ -257,11 +261,11 @@ struct xmd_engine {
         launch("k_cell_count", [&] { k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0); });
         launch("k_cell_fill", [&] { k_cell_fill<<<nb, BLK, 0, stream>>>(D, p); });
-        launch("k_nl_count", [&] { k_nl_count<<<nslot / NL_WAVES, BLK, 0, stream>>>(D, G, Ppos, p); });
+        launch("k_nl_count", [&] { k_nl_count<<<nl_grid(), BLK, 0, stream>>>(D, G, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 1); });
-        launch("k_nl_fill", [&] { k_nl_fill<<<nslot / NL_WAVES, BLK, 0, stream>>>(D, G, Ppos, p); });
+        launch("k_nl_fill", [&] { k_nl_fill<<<nl_grid(true), BLK, 0, stream>>>(D, G, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2); });
-        launch("k_fp_write", [&] { k_fp_write<<<nb, BLK, 0, stream>>>(D, p); });
+        launch("k_fp_write", [&] { k_fp_write<<<nl_grid(), BLK, 0, stream>>>(D, p); });
     }
     void step_seq(int p) {
         if (G.qa_on) {
@@ -452,6 +456,15 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Fail{XMD_ERR_NO_DEVICE, "no HIP device"};
     HIPCHK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+    {
+        int dev = 0, ncu = 0, bc = 0, bf = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bc, k_nl_count, BLK, 0));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bf, k_nl_fill, BLK, 0));
+        E.grid_count = std::max(1, bc) * std::max(1, ncu);
+        E.grid_fill = std::max(1, bf) * std::max(1, ncu);
+    }
     E.nrep = t->n_replicas;
     E.nbeads = t->n_beads;
     E.nchains = t->n_chains;
@@ -684,6 +697,7 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     D.nl_len = E.alloc<int>(n);
     D.nl_half = E.alloc<int>(n);
     D.fp_slice_cnt = E.alloc<int>(nsl);
+    D.fp_n = E.alloc<int>(n);
     D.fp_slice_start = E.alloc<int>(nsl);
     // QA contacts: at most MAXQ per bead -> 8n live; twice that for vacancies
     E.ct_base_h.resize(E.nrep + 1);
@@ -733,7 +747,7 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     k_cell_count<<<nb, BLK, 0, E.stream>>>(D, D.pos[0], 0);
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 0);
     k_cell_fill<<<nb, BLK, 0, E.stream>>>(D, 0);
-    k_nl_count<<<n / NL_WAVES, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
+    k_nl_count<<<E.nl_grid(), BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
     HIPCHK(hipGetLastError());
     auto len = E.get(D.nl_len, n);
     std::vector<int> w(nsl, 0);
@@ -785,9 +799,9 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     E.qa_grid = std::max(1, std::min(E.nfp_chunks + E.nct_chunks, 8192));
     // finish the first rebuild
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 1);
-    k_nl_fill<<<n / NL_WAVES, BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
+    k_nl_fill<<<E.nl_grid(true), BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 2);
-    k_fp_write<<<nb, BLK, 0, E.stream>>>(D, 0);
+    k_fp_write<<<E.nl_grid(), BLK, 0, E.stream>>>(D, 0);
     HIPCHK(hipGetLastError());
     std::fill(E.pending_rb.begin(), E.pending_rb.end(), 0);
     E.check_err();

@@ -66,14 +66,18 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
         int type = 0;
         uint32_t d1 = 0, d2 = 0;
         if (sift) {
-            if (local < count && !D.fp_vac[idx]) {
-                int i1 = D.fp_i1[idx], i2 = D.fp_i2[idx];
+            // entries below the live count hold valid slots even when vacant,
+            // so the pair, its vacancy flag and its positions load without
+            // waiting on the flag
+            if (local < count) {
+                const int i1 = D.fp_i1[idx], i2 = D.fp_i2[idx];
+                const uint8_t vac = D.fp_vac[idx];
                 v3<double> r1 = ldp(P, i1), r2 = ldp(P, i2);
                 v3<double> r12 = r_uv(box, r1, r2);
                 // qa_sift rejects a pair of type k when 1 > r12_rn req[k]; with
                 // req[k] <= req_max that holds for every k once it holds for
                 // req_max (rounding is monotone), so far pairs skip n and h
-                if (!(1.0 > norm_inv(r12) * G.qa.req_max)) {
+                if (!vac && !(1.0 > norm_inv(r12) * G.qa.req_max)) {
                     uint8_t f1 = D.bflags[i1], f2 = D.bflags[i2];
                     bool ok1 = (f1 & B_PREV) && (f1 & B_NEXT), ok2 = (f2 & B_PREV) && (f2 & B_NEXT);
                     v3<double> n1, h1, n2, h2;
@@ -617,7 +621,11 @@ __device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &
     return f;
 }
 
-// Wavefront per slot: the lanes share the slot's candidate partners.
+// Wavefront per slot: the lanes share the slot's candidate partners.  With
+// cells, the 27 neighbour cells (dx fastest, then dy, dz) form one flat
+// candidate sequence: lane c < 27 holds cell c's range, a wave scan gives
+// the offsets, and each lane finds the cell of its candidate by bisection
+// over them, so every iteration uses all 64 lanes.
 template <typename Fn> __device__ inline void for_candidates_wave(const Dev &D, int s, int rep, Fn fn) {
     const int l = lane_id();
     int c0 = D.cell_off[rep];
@@ -632,41 +640,64 @@ template <typename Fn> __device__ inline void for_candidates_wave(const Dev &D, 
     const int *nd = D.cell_dim + 3 * rep;
     int c = D.bead_cell[s] - c0;
     int cx = c % nd[0], cy = (c / nd[0]) % nd[1], cz = c / (nd[0] * nd[1]);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int x = (cx + dx + nd[0]) % nd[0], y = (cy + dy + nd[1]) % nd[1], z = (cz + dz + nd[2]) % nd[2];
-                int cc = c0 + (z * nd[1] + y) * nd[0] + x;
-                int a = D.cell_start[cc], b = D.cell_end[cc];
-                for (int k0 = a; k0 < b; k0 += WAVE) {
-                    int k = k0 + l;
-                    int j = k < b ? D.cell_beads[k] : -1;
-                    fn(j != s ? j : -1);
-                }
-            }
+    int start = 0, cnt = 0;
+    if (l < 27) {
+        int x = (cx + l % 3 - 1 + nd[0]) % nd[0], y = (cy + (l / 3) % 3 - 1 + nd[1]) % nd[1],
+            z = (cz + l / 9 - 1 + nd[2]) % nd[2];
+        int cc = c0 + (z * nd[1] + y) * nd[0] + x;
+        start = D.cell_start[cc];
+        cnt = D.cell_end[cc] - start;
+    }
+    int incl = cnt;
+    for (int o = 1; o < 32; o <<= 1) {
+        int v = __shfl_up(incl, o);
+        if (l >= o) incl += v;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, 26);
+    for (int k0 = 0; k0 < total; k0 += WAVE) {
+        const int k = k0 + l;
+        // largest cell q in [0, 27) with excl[q] <= k (excl[0] = 0)
+        int lo = 0, hi = 26;
+        for (int it = 0; it < 5; ++it) {
+            int mid = (lo + hi + 1) >> 1;
+            if (__shfl(excl, mid) <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int st = __shfl(start, lo), ex = __shfl(excl, lo);
+        const int j = k < total ? D.cell_beads[st + k - ex] : -1;
+        fn(j != s ? j : -1);
+    }
 }
 
 constexpr int NL_WAVES = BLK / WAVE;  // slots per block in the rebuild kernels
 constexpr int MAXSORT = 512;
 
-// Row length per slot (entries carrying any term flag).
+// Row length per slot (entries carrying any term flag).  The rebuild
+// kernels loop over slots with a grid of one resident round of blocks and
+// skip replicas that do not rebuild this step (the test is uniform over a block: its NL_WAVES slots
+// lie in one 64-aligned replica range), so a step where one replica
+// rebuilds launches no idle wavefronts for the others.
 __global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double *P, int p) {
     if (!D.any_rb[p]) return;
-    const int s = blockIdx.x * NL_WAVES + threadIdx.x / WAVE;
-    const int rep = D.slot_rep[s];
-    if (!D.rb_now[rep]) return;  // uniform: the block's slots share a replica
-    int cnt = 0;
-    if (D.bflags[s] & B_ACTIVE) {
-        const Box box = D.box[rep];
-        const v3<double> rs = ldp(P, s);
-        for_candidates_wave(D, s, rep, [&](int j) {
-            if (j >= 0 && (pair_flags(D, G, box, s, j, rs, P) & (F_PAULI | F_ES | F_PID | F_QAFREE))) ++cnt;
-        });
-    }
-    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
-    if (lane_id() == 0) {
-        D.nl_len[s] = cnt;
-        if ((s & 63) == 0) D.fp_slice_cnt[s >> 6] = 0;
+    for (int s0 = blockIdx.x * NL_WAVES; s0 < D.nslot; s0 += gridDim.x * NL_WAVES) {
+        const int s = s0 + threadIdx.x / WAVE;
+        const int rep = D.slot_rep[s];
+        if (!D.rb_now[rep]) continue;
+        int cnt = 0;
+        if (D.bflags[s] & B_ACTIVE) {
+            const Box box = D.box[rep];
+            const v3<double> rs = ldp(P, s);
+            for_candidates_wave(D, s, rep, [&](int j) {
+                if (j >= 0 && (pair_flags(D, G, box, s, j, rs, P) & (F_PAULI | F_ES | F_PID | F_QAFREE))) ++cnt;
+            });
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if (lane_id() == 0) {
+            D.nl_len[s] = cnt;
+            D.fp_n[s] = 0;
+            if ((s & 63) == 0) D.fp_slice_cnt[s >> 6] = 0;
+        }
     }
 }
 
@@ -674,17 +705,10 @@ __global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double
 // write it; count the half row and the QA free pairs; snapshot orig_r and
 // the Go activity.  Rows longer than MAXSORT are written unsorted and
 // insertion-sorted in place by lane 0.
-__global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double *P, int p) {
-    __shared__ int s_j[NL_WAVES][MAXSORT];
-    __shared__ uint8_t s_f[NL_WAVES][MAXSORT];
-    if (!D.any_rb[p]) return;
-    if (*D.err & E_NL_CAP) return;
-    const int w = threadIdx.x / WAVE, l = lane_id();
-    const int s = blockIdx.x * NL_WAVES + w;
-    const int rep = D.slot_rep[s];
-    if (!D.rb_now[rep]) return;  // uniform: the block's slots share a replica
+__device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double *P, int s, int rep, int *lj,
+                                    uint8_t *lf) {
+    const int l = lane_id();
     const bool active = D.bflags[s] & B_ACTIVE;
-    const int n = D.nslot;
     const Box box = D.box[rep];
     const v3<double> rs = ldp(P, s);
     const int base = D.slice_start[s >> 6] + (s & 63);
@@ -699,8 +723,8 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
             int pos = cnt + __popcll(m & ((1ull << l) - 1ull));
             if (keep) {
                 if (in_lds) {
-                    s_j[w][pos] = j;
-                    s_f[w][pos] = f;
+                    lj[pos] = j;
+                    lf[pos] = f;
                 } else {
                     D.nl_j[base + 64 * pos] = j;
                     D.nl_flag[base + 64 * pos] = f;
@@ -710,14 +734,14 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
             cnt += __popcll(m);
         });
     }
-    __syncthreads();
+    __syncthreads();  // reached by every wave of the block: the skip test is block-uniform
     int below = 0;
     if (active && in_lds) {
         for (int k = l; k < cnt; k += WAVE) {
-            int j = s_j[w][k], r = 0;
-            for (int q = 0; q < cnt; ++q) r += s_j[w][q] < j;
+            int j = lj[k], r = 0;
+            for (int q = 0; q < cnt; ++q) r += lj[q] < j;
             D.nl_j[base + 64 * r] = j;
-            D.nl_flag[base + 64 * r] = s_f[w][k];
+            D.nl_flag[base + 64 * r] = lf[k];
             below += j < s;
         }
     } else if (active && l == 0) {
@@ -742,6 +766,7 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
     }
     if (l == 0) {
         D.nl_half[s] = below;
+        D.fp_n[s] = nfp;
         if (nfp) atomicAdd(D.fp_slice_cnt + (s >> 6), nfp);
         stp(D.orig_r, s, rs);
     }
@@ -754,42 +779,56 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
     }
 }
 
-// Free pairs in canonical order; zero the cell counters; count the rebuild.
+__global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double *P, int p) {
+    __shared__ int s_j[NL_WAVES][MAXSORT];
+    __shared__ uint8_t s_f[NL_WAVES][MAXSORT];
+    if (!D.any_rb[p]) return;
+    if (*D.err & E_NL_CAP) return;
+    const int w = threadIdx.x / WAVE, l = lane_id();
+    for (int s0 = blockIdx.x * NL_WAVES; s0 < D.nslot; s0 += gridDim.x * NL_WAVES) {
+        const int s = s0 + w;
+        const int rep = D.slot_rep[s];
+        if (!D.rb_now[rep]) continue;  // uniform over the block
+        nl_fill_slot(D, G, P, s, rep, s_j[w], s_f[w]);
+    }
+}
+
+// Free pairs in canonical order (slot, then row order); zero the cell
+// counters; count the rebuild.  Wavefront per slot: the slot's pairs follow
+// those of the lower slots of its 64-slot slice.
 __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
     if (!D.any_rb[p]) return;
-    int s = blockIdx.x * BLK + threadIdx.x;
-    int rep = D.slot_rep[s];
-    if (!D.rb_now[rep]) return;
-    const int n = D.nslot;
-    int nfp = 0;
-    bool active = D.bflags[s] & B_ACTIVE;
-    int base = D.slice_start[s >> 6] + (s & 63);
-    int len = active ? D.nl_len[s] : 0;
-    for (int k = 0; k < len; ++k) nfp += (D.nl_flag[base + 64 * k] & F_QAFREE) != 0;
-    // exclusive prefix within the wavefront
-    int incl = nfp;
-    for (int o = 1; o < 64; o <<= 1) {
-        int x = __shfl_up(incl, o);
-        if (lane_id() >= o) incl += x;
-    }
-    int off = D.fp_slice_start[s >> 6] + incl - nfp;
-    int end = D.fp_base[rep] + D.fp_cap[rep];
-    for (int k = 0; k < len; ++k) {
-        int e = base + 64 * k;
-        if (D.nl_flag[e] & F_QAFREE) {
-            if (off < end) {
-                D.fp_i1[off] = s;
-                D.fp_i2[off] = D.nl_j[e];
-                D.fp_vac[off] = 0;
+    const int l = lane_id();
+    for (int s0 = blockIdx.x * NL_WAVES; s0 < D.nslot; s0 += gridDim.x * NL_WAVES) {
+        const int s = s0 + threadIdx.x / WAVE;
+        const int rep = D.slot_rep[s];
+        if (!D.rb_now[rep]) continue;
+        const int sl = s >> 6;
+        int before = l < (s & 63) ? D.fp_n[(sl << 6) + l] : 0;
+        for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o);
+        int off = D.fp_slice_start[sl] + before;
+        const int end = D.fp_base[rep] + D.fp_cap[rep];
+        const int base = D.slice_start[sl] + (s & 63);
+        const int len = (D.bflags[s] & B_ACTIVE) && D.fp_n[s] ? D.nl_len[s] : 0;
+        for (int k0 = 0; k0 < len; k0 += WAVE) {
+            const int k = k0 + l, e = base + 64 * k;
+            const bool keep = k < len && (D.nl_flag[e] & F_QAFREE);
+            const unsigned long long m = __ballot(keep);
+            const int o = off + __popcll(m & ((1ull << l) - 1ull));
+            if (keep && o < end) {
+                D.fp_i1[o] = s;
+                D.fp_i2[o] = D.nl_j[e];
+                D.fp_vac[o] = 0;
             }
-            ++off;
+            off += __popcll(m);
+        }
+        if (l == 0) {
+            const int local = s - D.rep_slot_off[rep];
+            const int c0 = D.cell_off[rep], nc = D.cell_off[rep + 1] - c0;
+            for (int c = local; c < nc; c += D.rep_n[rep]) D.cell_cnt[c0 + c] = 0;
+            if (local == 0) D.n_rebuild[rep] += 1;
         }
     }
-    int local = s - D.rep_slot_off[rep];
-    int c0 = D.cell_off[rep], nc = D.cell_off[rep + 1] - c0;
-    for (int c = local; c < nc; c += D.rep_n[rep]) D.cell_cnt[c0 + c] = 0;
-    if (local == 0) D.n_rebuild[rep] += 1;
-    (void)n;
 }
 
 // Per-bead count / fill of the full canonical Verlet list from orig_r
