@@ -24,9 +24,12 @@ template <typename T> static void pid_all(int n, const double *geo, const double
     for (int i = 0; i < n; ++i) {
         const double *g = geo + 27 * i;
         v3<T> o[6];
-        T v = pid_bundle<T>(P, (T)ss[3 * i], (T)ss[3 * i + 1], (T)ss[3 * i + 2], V<double>(g), V<double>(g + 3),
-                            V<double>(g + 6), V<double>(g + 9), V<double>(g + 12), V<double>(g + 15),
-                            V<double>(g + 18), V<double>(g + 21), V<double>(g + 24), o);
+        // geometry: side 1 bonds (g, g+3, g+6), side 2 bonds (g+12, g+15, g+18),
+        // r12 at g+24 (g+9 and g+21 repeat r12 and -r12)
+        T v = pid_bundle<T>(P, (T)ss[3 * i], (T)ss[3 * i + 1], (T)ss[3 * i + 2],
+                            pid_side<T>(V<double>(g), V<double>(g + 3), V<double>(g + 6)),
+                            pid_side<T>(V<double>(g + 12), V<double>(g + 15), V<double>(g + 18)),
+                            V<double>(g + 24), o);
         for (int k = 0; k < 6; ++k) {
             out[19 * i + 3 * k] = o[k].x;
             out[19 * i + 3 * k + 1] = o[k].y;

@@ -104,6 +104,14 @@ __device__ inline void st3(double *a, int n, int s, v3<double> v) {
 }
 __device__ inline int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by block index, and
+// each XCD has its own L2.  For a grid of nb blocks (a multiple of 8) this
+// maps block b to a position so that XCD x covers the contiguous eighth
+// [x nb/8, (x+1) nb/8) of the grid: the partner gathers of one XCD then
+// stay within a few replicas, whose positions fit its L2.
+constexpr int NUM_XCD = 8;
+__device__ inline int xcd_block(int b, int nb) { return (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD; }
+
 // Parameters of the force kernel in its precision T, and the term mask.
 template <typename T> struct TermTab {
     TermParams<T> P;

@@ -44,7 +44,7 @@ template <typename T, bool EVAL, int LPB>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
     const int n = D.nslot;
-    const int t = blockIdx.x * BLK + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     const int s = t / LPB, sub = t % LPB;
     const int rep = D.slot_rep[s];
     const uint8_t bf = D.bflags[s];
@@ -174,8 +174,9 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
                     const int i1 = owner ? s : j, i2 = owner ? j : s;
                     const int ty = 20 * D.atype[i1] + D.atype[i2];
                     v3<T> out[6];
-                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty], ri - rsp, rsn - ri,
-                                        rsn - rsp, dd, rj - rjp, rjn - rj, rjn - rjp, -dd, dd, out);
+                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty],
+                                        pid_side<T>(ri - rsp, rsn - ri, rsn - rsp),
+                                        pid_side<T>(rj - rjp, rjn - rj, rjn - rjp), dd, out);
                     add3(Fx, Fy, Fz, out[1]);
                     add3(tpx, tpy, tpz, out[0]);
                     add3(tnx, tny, tnz, out[2]);

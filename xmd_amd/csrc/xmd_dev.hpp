@@ -258,40 +258,61 @@ template <typename T> struct Psi {
     T psi;
     v3<T> g_prev, g_cen, g_next, g_par;
 };
-template <typename T>
-XMD_HD inline Psi<T> pid_psi(v3<double> r12d, v3<double> r23d, v3<double> r13d, v3<double> r24d) {
-    const v3<double> r14d = r12d + r24d;
-    const v3<double> rmd = cross(r23d, r12d), rnd = cross(r13d, r14d);
-    const v3<T> rm = cvt<T>(rmd), rn = cvt<T>(rnd);
-    const v3<T> rij = cvt<T>(-r23d), rkj = cvt<T>(-r13d), rkl = cvt<T>(-r14d);
-    const T rm_rn = norm_inv(rm), rn_rn = norm_inv(rn);
-    const T rkj_n = norm(rkj);
+// The partner-independent part of one side: its bonds (FP64), the plane
+// normal m = r23 x r12 and the scalars built from them, computed once per
+// bead and reused for every partner.
+template <typename T> struct PidSide {
+    v3<double> r12d, r13d, rmd;
+    v3<T> rm, rij, rkj, gi;
+    T rm_rn, rkj_n, inv2, pp;
+};
+template <typename T> XMD_HD inline PidSide<T> pid_side(v3<double> r12d, v3<double> r23d, v3<double> r13d) {
+    PidSide<T> S;
+    S.r12d = r12d;
+    S.r13d = r13d;
+    S.rmd = cross(r23d, r12d);
+    S.rm = cvt<T>(S.rmd);
+    S.rij = cvt<T>(-r23d);
+    S.rkj = cvt<T>(-r13d);
+    S.rm_rn = norm_inv(S.rm);
+    S.rkj_n = norm(S.rkj);
+    S.gi = S.rm * (S.rkj_n * S.rm_rn * S.rm_rn);
+    S.inv2 = rcpT(S.rkj_n * S.rkj_n);
+    S.pp = dot(S.rij, S.rkj) * S.inv2;
+    return S;
+}
+// psi of side S against the partner at r24 = wrap(partner - cen), and its
+// gradient.
+template <typename T> XMD_HD inline Psi<T> pid_psi(const PidSide<T> &S, v3<double> r24d) {
+    const v3<double> r14d = S.r12d + r24d;
+    const v3<double> rnd = cross(S.r13d, r14d);
+    const v3<T> rn = cvt<T>(rnd);
+    const v3<T> rkl = cvt<T>(-r14d);
+    const T rn_rn = norm_inv(rn);
     T psi;
     if constexpr (std::is_same<T, double>::value) {
-        double cos_psi = dot(rm, rn) * rm_rn * rn_rn;
+        double cos_psi = dot(S.rm, rn) * S.rm_rn * rn_rn;
         psi = acos_c(cos_psi);
-        if (dot(rij, rn) < 0.0) psi = -psi;
+        if (dot(S.rij, rn) < 0.0) psi = -psi;
     } else {
         // sin psi |m||n| = (m.r_kl)|r_kj|: (m x n) = (m.r_kl) r_kj, r_ij.n = m.r_kl;
         // m.r_kl cancels like the cross products, so it is formed in FP64
-        psi = atan2_f((float)dot(rmd, -r14d) * rkj_n, dot(rm, rn));
+        psi = atan2_f((float)dot(S.rmd, -r14d) * S.rkj_n, dot(S.rm, rn));
     }
-    const v3<T> gi = rm * (rkj_n * rm_rn * rm_rn);
-    const v3<T> gl = -(rn * (rkj_n * rn_rn * rn_rn));
-    const T inv2 = rcpT(rkj_n * rkj_n);
-    const T pp = dot(rij, rkj) * inv2, qq = dot(rkl, rkj) * inv2;
-    const v3<T> gj = (pp - T(1.0f)) * gi - qq * gl;
-    const v3<T> gk = (qq - T(1.0f)) * gl - pp * gi;
-    return {psi, gk, gi, gj, gl};
+    const v3<T> gl = -(rn * (S.rkj_n * rn_rn * rn_rn));
+    const T qq = dot(rkl, S.rkj) * S.inv2;
+    const v3<T> gj = (S.pp - T(1.0f)) * S.gi - qq * gl;
+    const v3<T> gk = (qq - T(1.0f)) * gl - S.pp * S.gi;
+    return {psi, gk, S.gi, gj, gl};
 }
-// Full PID bundle (eval_forces.cpp:93-140, +D8).  Geometry of the canonical
-// pair (i1 < i2).  out[0..5] = forces on i1p, i1, i1n, i2p, i2, i2n.
+// Full PID bundle (eval_forces.cpp:93-140, +D8): sides S1 (bead i1) and S2
+// (bead i2), r12 = wrap(r_i2 - r_i1).  out[0..5] = forces on i1p, i1, i1n,
+// i2p, i2, i2n.
 template <typename T>
-XMD_HD inline T pid_bundle(const TermParams<T> &P, T ss_depth, T ss_rmin, T ss_rmax, v3<double> a12, v3<double> a23,
-                           v3<double> a13, v3<double> a24, v3<double> b12, v3<double> b23, v3<double> b13,
-                           v3<double> b24, v3<double> r12d, v3<T> out[6]) {
-    Psi<T> g1 = pid_psi<T>(a12, a23, a13, a24);
-    Psi<T> g2 = pid_psi<T>(b12, b23, b13, b24);
+XMD_HD inline T pid_bundle(const TermParams<T> &P, T ss_depth, T ss_rmin, T ss_rmax, const PidSide<T> &S1,
+                           const PidSide<T> &S2, v3<double> r12d, v3<T> out[6]) {
+    Psi<T> g1 = pid_psi<T>(S1, r12d);
+    Psi<T> g2 = pid_psi<T>(S2, -r12d);
     T psi1 = g1.psi, psi2 = g2.psi;
     v3<T> r12 = cvt<T>(r12d);
     T r12_n = norm(r12), r12_rn = rcpT(r12_n);
