@@ -46,22 +46,37 @@ def row_entries(eng):
     return eng.counters()["nl_entries"]
 
 
-def force_kernel_bytes(eng, system, fused):
+def force_kernel_bytes(eng, system):
     """Algorithmic bytes of one launch of the per-bead force kernel.
 
     Per neighbour-row entry: int32 partner + uint8 flags (5 B).  Per native
     row entry: int32 partner + fp64 distance + uint8 active (13 B).  Per
-    bead: its position (24 B; partner gathers re-read positions that are
-    already counted), bonded natives (5 fp64 = 40 B), row length + slice
-    offset (8 B), flags + aa type (2 B), chain id + QA slot count (8 B),
-    energy write (8 B), and either the Gear state (y1..y5 read and write,
-    y0 write, orig_r read: 288 B; fused form) or the force and the PID
-    spill to the chain neighbours (24 + 48 B; split form)."""
+    bead: its position (32 B, AoS x/y/z/pad; partner gathers re-read
+    positions that are already counted), bonded natives (5 fp64 = 40 B),
+    row length + slice offset (8 B), flags + aa type (2 B), chain id + QA
+    slot count (8 B), energy write (8 B), force write (24 B) and the PID
+    spill to the chain neighbours (48 B)."""
     n = system.n_beads
     e = row_entries(eng)
     nnat = 2 * len(system.nat_i1)
-    per_bead = 24 + 40 + 8 + 2 + 8 + 8 + (288 if fused else 72)
+    per_bead = 32 + 40 + 8 + 2 + 8 + 8 + 24 + 48
     return 5 * e + 13 * nnat + per_bead * n
+
+
+def measured_traffic(kernel, replicas):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
+    copy of known size), when it was taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or t.get("replicas") != replicas:
+        return None
+    return int(k["hbm_bytes_per_launch"])
 
 
 def copy_calibration(mib):
@@ -122,21 +137,28 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (launches, total_ms) = dom
     avg_ms = total_ms / launches
-    nbytes = force_kernel_bytes(eng, batch, False) if name.startswith("k_force") else None
+    nbytes = force_kernel_bytes(eng, batch) if name.startswith("k_force") else None
     roofline = None
     if nbytes is not None:
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-                    "avg_ms": round(avg_ms, 5), "bytes_per_launch": int(nbytes)}
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, args.replicas),
+                    "kernel": name, "avg_ms": round(avg_ms, 5), "bytes_per_launch": int(nbytes)}
     kernels = {k: {"launches": n, "avg_us": round(1e3 * ms / n, 2), "share": round(ms / sum(v[1] for v in prof.values()), 4)}
                for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     log("kernels:", json.dumps(kernels))
 
     # ---- plain FP64 on one box, and the mixed force error against it
     fp64_value = None
+    fp64_batched = None
     rel_rms = None
     if not args.no_fp64:
+        eb = Engine(p_fp64, batch)
+        kb = max(100, args.steps // 10)
+        dtb = timed_steps(eb, min(args.warmup, 50), kb)
+        fp64_batched = batch.n_beads * kb / dtb
+        log(f"fp64 batched: {kb} steps in {dtb:.3f}s -> {fp64_batched:.3e} bead-steps/s")
+        eb.close()
         e64 = Engine(p_fp64, single)
         k64 = max(200, args.steps // 4)
         dt64 = timed_steps(e64, min(args.warmup, 100), k64)
@@ -196,6 +218,8 @@ def main():
                    "seeds": [int(x) for x in batch.rng_state]},
         "fp64_single_box": None if fp64_value is None else round(fp64_value, 1),
         "mixed_over_fp64": None if fp64_value is None else round(value / fp64_value, 3),
+        "fp64_batched": None if fp64_batched is None else round(fp64_batched, 1),
+        "mixed_over_fp64_batched": None if fp64_batched is None else round(value / fp64_batched, 3),
         "force_rel_rms_mixed_vs_fp64": rel_rms,
         "rebuilds": counters["rebuilds"],
         "nl_entries": counters["nl_entries"],
