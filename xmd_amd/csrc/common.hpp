@@ -57,8 +57,8 @@ struct Dev {
     const double *nat_d;
     uint8_t *nat_act;
     // neighbour rows
-    const int *nl_base, *nl_cap;
-    int *slice_start, *slice_w, *nl_len, *nl_half, *nl_j;
+    int nl_w;  // row capacity per slot (rows are fixed-width sliced ELL)
+    int *slice_start, *nl_len, *nl_half, *nl_j;
     uint8_t *nl_flag;
     // QA free pairs, candidates
     const int *fp_base, *fp_cap, *fp_chunk_rep;

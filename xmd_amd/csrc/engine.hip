@@ -160,8 +160,9 @@ struct xmd_engine {
     int chunk_steps = 16;
     int64_t steps = 0;
     std::vector<char> pending_rb;
-    std::vector<int> nl_base_h, nl_cap_h, fp_base_h, fp_cap_h, ct_base_h, ct_cap_h;
-    int nl_total = 0, fp_total = 0, ct_total = 0;
+    std::vector<int> fp_base_h, fp_cap_h, ct_base_h, ct_cap_h;
+    long long nl_total = 0;
+    int fp_total = 0, ct_total = 0, nl_w = 0;
     // per-kernel event timing (eager profiling pass only)
     bool prof_on = false;
     std::vector<std::string> prof_names;
@@ -250,6 +251,7 @@ struct xmd_engine {
     }
 
     int nblk() const { return nslot / BLK; }
+    size_t qa_lds() const { return (2 * (size_t)nrep + 1) * sizeof(int); }
     // one resident round of blocks of the grid-stride rebuild kernels (a
     // partial second round would leave most of the chip idle on a full rebuild)
     int grid_count = 0, grid_fill = 0;
@@ -261,15 +263,13 @@ struct xmd_engine {
         launch("k_cell_count", [&] { k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0); });
         launch("k_cell_fill", [&] { k_cell_fill<<<nb, BLK, 0, stream>>>(D, p); });
-        launch("k_nl_count", [&] { k_nl_count<<<nl_grid(), BLK, 0, stream>>>(D, G, Ppos, p); });
-        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 1); });
         launch("k_nl_fill", [&] { k_nl_fill<<<nl_grid(true), BLK, 0, stream>>>(D, G, Ppos, p); });
         launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2); });
         launch("k_fp_write", [&] { k_fp_write<<<nl_grid(), BLK, 0, stream>>>(D, p); });
     }
     void step_seq(int p) {
         if (G.qa_on) {
-            launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, 0, stream>>>(D, G, p, nfp_chunks, nct_chunks, 1); });
+            launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, qa_lds(), stream>>>(D, G, p, nfp_chunks, nct_chunks, 1); });
             launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
         }
         launch("k_force", [&] {
@@ -693,7 +693,6 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     if (P.rng_exact) D.bm_fix = E.upload(fix_table());
     int nsl = n / WAVE;
     D.slice_start = E.alloc<int>(nsl);
-    D.slice_w = E.alloc<int>(nsl);
     D.nl_len = E.alloc<int>(n);
     D.nl_half = E.alloc<int>(n);
     D.fp_slice_cnt = E.alloc<int>(nsl);
@@ -737,8 +736,6 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     E.put(D.force_rb, std::vector<int>(E.nrep, 1));
     // placeholders so kernels see valid pointers for the count pass
     std::vector<int> zero_base(E.nrep + 1, 0);
-    D.nl_base = E.upload(zero_base);
-    D.nl_cap = E.upload(zero_base);
     D.fp_base = E.upload(zero_base);
     D.fp_cap = E.upload(zero_base);
     D.fp_count = E.alloc<int>(E.nrep);
@@ -751,9 +748,22 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     HIPCHK(hipGetLastError());
     auto len = E.get(D.nl_len, n);
     std::vector<int> w(nsl, 0);
-    for (int s = 0; s < n; ++s) w[s / WAVE] = std::max(w[s / WAVE], len[s]);
-    E.nl_base_h.resize(E.nrep + 1);
-    E.nl_cap_h.resize(E.nrep + 1);
+    int maxlen = 0;
+    for (int s = 0; s < n; ++s) {
+        w[s / WAVE] = std::max(w[s / WAVE], len[s]);
+        maxlen = std::max(maxlen, len[s]);
+    }
+    // rows are fixed-width: three times the longest first row plus 64
+    // (chains compact after the start), at least 256, at most the LDS sort
+    // buffer; a longer row later is a capacity error
+    if (maxlen > MAXSORT) throw Fail{XMD_ERR_CAPACITY, "a neighbour row is longer than the largest supported row"};
+    E.nl_w = std::min(MAXSORT, std::max(256, round_up(3 * maxlen + 64, WAVE)));
+    D.nl_w = E.nl_w;
+    {
+        std::vector<int> ss(nsl);
+        for (int sl = 0; sl < nsl; ++sl) ss[sl] = sl * WAVE * E.nl_w;
+        E.put(D.slice_start, ss);
+    }
     E.fp_base_h.resize(E.nrep + 1);
     E.fp_cap_h.resize(E.nrep + 1);
     long long nl = 0, fp = 0;
@@ -762,23 +772,17 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
         for (int sl = E.rep_slot_off[k] / WAVE; sl < (E.rep_slot_off[k] + E.rep_n[k] + WAVE - 1) / WAVE; ++sl)
             need += 64LL * w[sl];
         long long cap = round_up((int)std::min<long long>(2 * need + 64LL * 64, 1LL << 30), BLK);
-        E.nl_base_h[k] = (int)nl;
-        E.nl_cap_h[k] = (int)cap;
-        nl += cap;
         long long fcap = P.qa_enabled ? round_up((int)(cap / 2 + E.ct_cap_h[k]), BLK) : BLK;
         E.fp_base_h[k] = (int)fp;
         E.fp_cap_h[k] = (int)fcap;
         fp += fcap;
     }
+    nl = (long long)n * E.nl_w;
     if (nl >= (1LL << 31) || fp >= (1LL << 31)) throw Fail{XMD_ERR_CAPACITY, "system too large for int32 list offsets"};
-    E.nl_base_h[E.nrep] = (int)nl;
-    E.nl_cap_h[E.nrep] = (int)nl;
     E.fp_base_h[E.nrep] = (int)fp;
     E.fp_cap_h[E.nrep] = (int)fp;
-    E.nl_total = (int)nl;
+    E.nl_total = nl;
     E.fp_total = (int)fp;
-    E.put(const_cast<int *>(D.nl_base), E.nl_base_h);
-    E.put(const_cast<int *>(D.nl_cap), E.nl_cap_h);
     E.put(const_cast<int *>(D.fp_base), E.fp_base_h);
     E.put(const_cast<int *>(D.fp_cap), E.fp_cap_h);
     D.nl_j = E.alloc<int>(nl);
@@ -797,8 +801,9 @@ void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     D.fp_chunk_rep = E.upload(fprep);
     E.nfp_chunks = (int)(fp / BLK);
     E.qa_grid = std::max(1, std::min(E.nfp_chunks + E.nct_chunks, 8192));
+    if (P.qa_enabled && E.qa_lds() > 60 * 1024)
+        throw Fail{XMD_ERR_CAPACITY, "too many replicas for one engine with QA enabled (at most 7680)"};
     // finish the first rebuild
-    k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 1);
     k_nl_fill<<<E.nl_grid(true), BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
     k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 2);
     k_fp_write<<<E.nl_grid(), BLK, 0, E.stream>>>(D, 0);
@@ -970,12 +975,14 @@ xmd_status xmd_get_list(xmd_engine *e, int32_t rep, int32_t id, int32_t *i1, int
         } else if (id == XMD_LIST_PAULI || id == XMD_LIST_ES || id == XMD_LIST_PID) {
             uint8_t want = id == XMD_LIST_PAULI ? F_PAULI : (id == XMD_LIST_ES ? F_ES : F_PID);
             auto len = e->get(e->D.nl_len, n);
-            auto ss = e->get(e->D.slice_start, n / WAVE);
-            auto j = e->get(e->D.nl_j, e->nl_total);
-            auto fl = e->get(e->D.nl_flag, e->nl_total);
+            // the replica's rows: slots [s0, s0 + nb) -> slices of width 64 nl_w
+            const size_t r0 = (size_t)(s0 / WAVE) * WAVE * e->nl_w;
+            const size_t rn = (size_t)((nb + WAVE - 1) / WAVE) * WAVE * e->nl_w;
+            auto j = e->get(e->D.nl_j + r0, rn);
+            auto fl = e->get(e->D.nl_flag + r0, rn);
             for (int s = s0; s < s0 + nb; ++s)
                 for (int k = 0; k < len[s]; ++k) {
-                    int idx = ss[s / WAVE] + 64 * k + s % WAVE;
+                    size_t idx = (size_t)(s / WAVE) * WAVE * e->nl_w - r0 + 64 * (size_t)k + s % WAVE;
                     if (j[idx] > s && (fl[idx] & want)) out.push_back({local(s), local(j[idx])});
                 }
         } else if (id == XMD_LIST_QA_FREE) {
@@ -1005,7 +1012,7 @@ xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double 
         uint32_t m = term_mask & e->mask;
         int p = e->cur;
         if (e->G.qa_on && (m & (1u << T_QA)))
-            k_qa_pairs<<<e->qa_grid, BLK, 0, e->stream>>>(e->D, e->G, p, e->nfp_chunks, e->nct_chunks, 0);
+            k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, p, e->nfp_chunks, e->nct_chunks, 0);
         int nb = e->nblk();
         TermTab<float> Tf = e->TTf;
         TermTab<double> Td = e->TTd;
@@ -1039,7 +1046,7 @@ xmd_status xmd_qa_update(xmd_engine *e) {
     return run_guarded(e->err, [&]() {
         e->ensure_lists();
         if (!e->G.qa_on) return;
-        k_qa_pairs<<<e->qa_grid, BLK, 0, e->stream>>>(e->D, e->G, e->cur, e->nfp_chunks, e->nct_chunks, 1);
+        k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, e->cur, e->nfp_chunks, e->nct_chunks, 1);
         k_qa_serial<<<e->nrep, BLK, 0, e->stream>>>(e->D, e->G, e->cur);
         HIPCHK(hipGetLastError());
         e->check_err();

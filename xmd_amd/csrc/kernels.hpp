@@ -42,15 +42,53 @@ __device__ inline uint64_t wave_lane_state(const Dev &D, uint64_t S, int w) {
 // Chunks [0, nfp_chunks): sift 256 free pairs each (qa/sift_candidates.cpp);
 // chunks after that: process 256 contacts each (qa/process_contacts.cpp).
 // Candidates and removals are compacted per chunk in index order, so the
-// serial pass sees them in set order without sorting.  Grid-stride over
-// chunks; chunks past a replica's live count are skipped.
+// serial pass sees them in set order without sorting.  Grid-stride over the
+// live chunks only (those below each replica's live counts).
 __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int nfp_chunks, int nct_chunks, int mutate) {
+    // s_pre (2 nrep + 1 ints, dynamic): prefix over the segments "free pairs
+    // of replica r" (r < nrep, only when sifting) and "contacts of replica
+    // r" of their live chunk counts; block wi of the work sits in segment g
+    // with s_pre[g] <= wi < s_pre[g+1].
+    extern __shared__ int s_pre[];
     __shared__ int wsum[BLK / WAVE];
     const int tid = threadIdx.x;
+    const int nseg = 2 * D.nrep;
+    int nwork = 0;
+    for (int g0 = 0; g0 < nseg; g0 += BLK) {
+        const int g = g0 + tid;
+        int live = 0;
+        if (g < D.nrep) live = mutate ? (D.fp_count[g] + BLK - 1) / BLK : 0;
+        else if (g < nseg) live = (D.ct_count[g - D.nrep] + BLK - 1) / BLK;
+        int x = live;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            int y = __shfl_up(x, o);
+            if (lane_id() >= o) x += y;
+        }
+        if (lane_id() == WAVE - 1) wsum[tid / WAVE] = x;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int k = 0; k < BLK / WAVE; ++k) {
+            if (k < tid / WAVE) woff += wsum[k];
+            tot += wsum[k];
+        }
+        if (g < nseg) s_pre[g + 1] = nwork + woff + x;
+        __syncthreads();
+        nwork += tot;
+    }
+    if (tid == 0) s_pre[0] = 0;
+    __syncthreads();
     const int n = D.nslot;
     const double *P = D.pos[p];
     const int ct_tot = D.ct_cap[D.nrep];
-    for (int blk = blockIdx.x; blk < nfp_chunks + nct_chunks; blk += gridDim.x) {
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        int lo = 0, hi = nseg - 1;  // largest g with s_pre[g] <= wi (a non-empty segment)
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pre[mid] <= wi) lo = mid;
+            else hi = mid - 1;
+        }
+        const int blk = lo < D.nrep ? D.fp_base[lo] / BLK + (wi - s_pre[lo])
+                                    : nfp_chunks + D.ct_base[lo - D.nrep] / BLK + (wi - s_pre[lo]);
         bool sift = blk < nfp_chunks;
         if (sift && !mutate) continue;
         int chunk = sift ? blk : blk - nfp_chunks;
@@ -505,7 +543,6 @@ __global__ __launch_bounds__(BLK) void k_cell_count(Dev D, const double *P, int 
 
 // One workgroup per replica: exclusive scan of a per-replica segment.
 // mode 0: cells (cell_cnt -> cell_start/end/cursor, absolute slot offsets);
-// mode 1: row slices (64*slice_w -> slice_start, capacity check);
 // mode 2: free-pair slices (fp_slice_cnt -> fp_slice_start, fp_count).
 __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
     __shared__ int s_buf[1024];
@@ -521,7 +558,7 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
     } else {
         lo = D.rep_slot_off[rep] >> 6;
         hi = (D.rep_slot_off[rep] + D.rep_n[rep] + 63) >> 6;
-        base = mode == 1 ? D.nl_base[rep] : D.fp_base[rep];
+        base = D.fp_base[rep];
     }
     if (tid == 0) s_carry = base;
     __syncthreads();
@@ -529,14 +566,7 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
         int c = c0 + tid;
         int v = 0;
         if (c < hi) {
-            if (mode == 0) v = D.cell_cnt[c];
-            else if (mode == 2) v = D.fp_slice_cnt[c];
-            else {
-                int wmax = 0;
-                for (int q = 0; q < 64; ++q) wmax = max(wmax, D.nl_len[64 * c + q]);
-                D.slice_w[c] = wmax;
-                v = 64 * wmax;
-            }
+            v = mode == 0 ? D.cell_cnt[c] : D.fp_slice_cnt[c];
         }
         s_buf[tid] = v;
         __syncthreads();
@@ -552,8 +582,6 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
                 D.cell_start[c] = ex;
                 D.cell_end[c] = ex + v;
                 D.cell_cursor[c] = ex;
-            } else if (mode == 1) {
-                D.slice_start[c] = ex;
             } else {
                 D.fp_slice_start[c] = ex;
             }
@@ -564,7 +592,6 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
     }
     if (tid == 0) {
         int used = s_carry - base;
-        if (mode == 1 && used > D.nl_cap[rep]) atomicOr(D.err, E_NL_CAP);
         if (mode == 2) {
             if (used > D.fp_cap[rep]) atomicOr(D.err, E_FP_CAP);
             D.fp_count[rep] = used > D.fp_cap[rep] ? D.fp_cap[rep] : used;
@@ -671,9 +698,10 @@ template <typename Fn> __device__ inline void for_candidates_wave(const Dev &D, 
 }
 
 constexpr int NL_WAVES = BLK / WAVE;  // slots per block in the rebuild kernels
-constexpr int MAXSORT = 512;
+constexpr int MAXSORT = 1024;  // longest row a slot may hold (the row width is at most this)
 
-// Row length per slot (entries carrying any term flag).  The rebuild
+// Row length per slot (entries carrying any term flag); run once at engine
+// creation to size the rows.  The rebuild
 // kernels loop over slots with a grid of one resident round of blocks and
 // skip replicas that do not rebuild this step (the test is uniform over a block: its NL_WAVES slots
 // lie in one 64-aligned replica range), so a step where one replica
@@ -693,18 +721,14 @@ __global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double
             });
         }
         for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
-        if (lane_id() == 0) {
-            D.nl_len[s] = cnt;
-            D.fp_n[s] = 0;
-            if ((s & 63) == 0) D.fp_slice_cnt[s >> 6] = 0;
-        }
+        if (lane_id() == 0) D.nl_len[s] = cnt;
     }
 }
 
-// Fill each row in LDS, rank-sort it by partner (partners are distinct),
-// write it; count the half row and the QA free pairs; snapshot orig_r and
-// the Go activity.  Rows longer than MAXSORT are written unsorted and
-// insertion-sorted in place by lane 0.
+// Gather each row in LDS (at most the row width; longer rows flag a
+// capacity error), rank-sort it by partner (partners are distinct), write
+// it; count the half row and the QA free pairs; snapshot orig_r and the Go
+// activity.
 __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double *P, int s, int rep, int *lj,
                                     uint8_t *lf) {
     const int l = lane_id();
@@ -712,8 +736,7 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
     const Box box = D.box[rep];
     const v3<double> rs = ldp(P, s);
     const int base = D.slice_start[s >> 6] + (s & 63);
-    const int len = active ? D.nl_len[s] : 0;
-    const bool in_lds = len <= MAXSORT;
+    const int W = D.nl_w;
     int cnt = 0, nfp = 0;
     if (active) {
         for_candidates_wave(D, s, rep, [&](int j) {
@@ -721,50 +744,43 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
             bool keep = f & (F_PAULI | F_ES | F_PID | F_QAFREE);
             unsigned long long m = __ballot(keep);
             int pos = cnt + __popcll(m & ((1ull << l) - 1ull));
-            if (keep) {
-                if (in_lds) {
-                    lj[pos] = j;
-                    lf[pos] = f;
-                } else {
-                    D.nl_j[base + 64 * pos] = j;
-                    D.nl_flag[base + 64 * pos] = f;
-                }
+            if (keep && pos < W) {
+                lj[pos] = j;
+                lf[pos] = f;
                 nfp += (f & F_QAFREE) != 0;
             }
             cnt += __popcll(m);
         });
+        if (cnt > W) {
+            if (l == 0) atomicOr(D.err, E_NL_CAP);
+            cnt = W;
+        }
     }
     __syncthreads();  // reached by every wave of the block: the skip test is block-uniform
     int below = 0;
-    if (active && in_lds) {
+    if (active) {
         for (int k = l; k < cnt; k += WAVE) {
-            int j = lj[k], r = 0;
+            const int j = lj[k];
+            int r = 0;
             for (int q = 0; q < cnt; ++q) r += lj[q] < j;
             D.nl_j[base + 64 * r] = j;
             D.nl_flag[base + 64 * r] = lf[k];
             below += j < s;
         }
-    } else if (active && l == 0) {
-        for (int a = 1; a < cnt; ++a) {
-            int j = D.nl_j[base + 64 * a];
-            uint8_t f = D.nl_flag[base + 64 * a];
-            int k = a;
-            while (k > 0 && D.nl_j[base + 64 * (k - 1)] > j) {
-                D.nl_j[base + 64 * k] = D.nl_j[base + 64 * (k - 1)];
-                D.nl_flag[base + 64 * k] = D.nl_flag[base + 64 * (k - 1)];
-                --k;
-            }
-            D.nl_j[base + 64 * k] = j;
-            D.nl_flag[base + 64 * k] = f;
-        }
-        for (int k = 0; k < cnt; ++k) below += D.nl_j[base + 64 * k] < s;
     }
-    if (!active) return;
+    if (!active) {
+        if (l == 0) {
+            D.nl_len[s] = 0;
+            D.fp_n[s] = 0;
+        }
+        return;
+    }
     for (int o = 32; o > 0; o >>= 1) {
         below += __shfl_xor(below, o);
         nfp += __shfl_xor(nfp, o);
     }
     if (l == 0) {
+        D.nl_len[s] = cnt;
         D.nl_half[s] = below;
         D.fp_n[s] = nfp;
         if (nfp) atomicAdd(D.fp_slice_cnt + (s >> 6), nfp);
@@ -784,7 +800,7 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
     __shared__ uint8_t s_f[NL_WAVES][MAXSORT];
     if (!D.any_rb[p]) return;
     if (*D.err & E_NL_CAP) return;
-    const int w = threadIdx.x / WAVE, l = lane_id();
+    const int w = threadIdx.x / WAVE;
     for (int s0 = blockIdx.x * NL_WAVES; s0 < D.nslot; s0 += gridDim.x * NL_WAVES) {
         const int s = s0 + w;
         const int rep = D.slot_rep[s];
@@ -823,6 +839,7 @@ __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
             off += __popcll(m);
         }
         if (l == 0) {
+            if ((s & 63) == 0) D.fp_slice_cnt[sl] = 0;  // consumed by k_seg_scan; zero for the next rebuild
             const int local = s - D.rep_slot_off[rep];
             const int c0 = D.cell_off[rep], nc = D.cell_off[rep + 1] - c0;
             for (int c = local; c < nc; c += D.rep_n[rep]) D.cell_cnt[c0 + c] = 0;
