@@ -63,10 +63,10 @@ def force_kernel_bytes(eng, system):
     return 5 * e + 13 * nnat + per_bead * n
 
 
-def measured_traffic(kernel, replicas):
+def measured_traffic(kernel, replicas, groups):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
-    copy of known size), when it was taken on this workload."""
+    copy of known size), when it was taken on this workload and split."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -74,7 +74,7 @@ def measured_traffic(kernel, replicas):
     except (OSError, ValueError):
         return None
     k = t.get("kernels", {}).get(kernel)
-    if not k or t.get("replicas") != replicas:
+    if not k or t.get("replicas") != replicas or t.get("groups", 1) != groups:
         return None
     return int(k["hbm_bytes_per_launch"])
 
@@ -137,12 +137,16 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (launches, total_ms) = dom
     avg_ms = total_ms / launches
-    nbytes = force_kernel_bytes(eng, batch) if name.startswith("k_force") else None
+    # the replicas are stepped as stream groups, one launch per group per
+    # step: bytes of one launch = the batch's bytes / launches per step
+    per_step = launches / args.profile_steps
+    nbytes = force_kernel_bytes(eng, batch) / per_step if name.startswith("k_force") else None
     roofline = None
     if nbytes is not None:
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, args.replicas),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(name, args.replicas, round(per_step)),
                     "kernel": name, "avg_ms": round(avg_ms, 5), "bytes_per_launch": int(nbytes)}
     kernels = {k: {"launches": n, "avg_us": round(1e3 * ms / n, 2), "share": round(ms / sum(v[1] for v in prof.values()), 4)}
                for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
@@ -213,7 +217,8 @@ def main():
         "dtype": "fp32 forces / fp64 state",
         "data": "synthetic (seeded self-avoiding chains)",
         "config": {"workload": f"C2 box 40x250 beads (10k, PBC L=464.2 A, full term set) x {args.replicas} "
-                               f"replicas, mixed precision, graph-captured step",
+                               f"replicas, mixed precision, graph-captured step, replicas in "
+                               f"{round(per_step)} concurrent stream groups",
                    "replicas": args.replicas, "beads_total": batch.n_beads, "parallelism": "replicas in one launch",
                    "seeds": [int(x) for x in batch.rng_state]},
         "fp64_single_box": None if fp64_value is None else round(fp64_value, 1),

@@ -97,7 +97,10 @@ typedef struct xmd_params {
     int32_t langevin_enabled;
     int32_t precision;          /* xmd_precision */
     int32_t rng_exact;          /* 1: noise bit-equal to glibc Box-Muller */
-    int32_t _pad0;
+    int32_t stream_groups;      /* replica groups stepped on concurrent HIP
+                                   streams: 0 = automatic (2 when there are
+                                   at least 2 replicas), n = min(n, replicas);
+                                   no counterpart in freeform_params */
 
     /* neighbour list ("neighbor list") */
     double pad_factor;

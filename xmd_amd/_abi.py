@@ -34,7 +34,7 @@ _d, _i32, _i8 = C.c_double, C.c_int32, C.c_int8
 class XmdParams(C.Structure):
     _fields_ = [
         ("dt", _d), ("gamma_factor", _d),
-        ("langevin_enabled", _i32), ("precision", _i32), ("rng_exact", _i32), ("_pad0", _i32),
+        ("langevin_enabled", _i32), ("precision", _i32), ("rng_exact", _i32), ("stream_groups", _i32),
         ("pad_factor", _d), ("max_cutoff_override", _d),
         ("tether_enabled", _i32), ("_pad1", _i32),
         ("tether_H1", _d), ("tether_H2", _d), ("tether_def_length", _d),

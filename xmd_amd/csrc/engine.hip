@@ -141,7 +141,9 @@ int round_up(int x, int m) { return (x + m - 1) / m * m; }
 }  // namespace
 
 // ================================================================ engine ===
-struct xmd_engine {
+// One part of an engine: device state and captured step graphs for a
+// contiguous range of replicas, on its own HIP stream.
+struct Part {
     xmd_params P;
     int nrep = 0, nbeads = 0, nslot = 0, nchains = 0;
     std::vector<int> rep_off, rep_slot_off, rep_n, slot_of_bead, bead_of_slot, slot_rep_h;
@@ -239,7 +241,7 @@ struct xmd_engine {
         HIPCHK(hipStreamSynchronize(stream));
         return h;
     }
-    ~xmd_engine() {
+    ~Part() {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         for (auto &g : g_chunk)
             if (g) (void)hipGraphExecDestroy(g);
@@ -316,31 +318,28 @@ struct xmd_engine {
             throw Fail{XMD_ERR_CAPACITY, m};
         }
     }
-    void run_steps(int64_t n) {
-        ensure_lists();
-        while (n > 0) {
-            if (n >= chunk_steps) {
-                if (!g_chunk[cur]) g_chunk[cur] = capture(cur, chunk_steps);
-                HIPCHK(hipGraphLaunch(g_chunk[cur], stream));
-                if (chunk_steps & 1) cur ^= 1;
-                n -= chunk_steps;
-                steps += chunk_steps;
-            } else {
-                if (!g_one[cur]) g_one[cur] = capture(cur, 1);
-                HIPCHK(hipGraphLaunch(g_one[cur], stream));
-                cur ^= 1;
-                n -= 1;
-                steps += 1;
-            }
+    // Launches the graph for the next min(n, chunk_steps) steps (no wait);
+    // returns how many steps it covers.
+    int64_t launch_next(int64_t n) {
+        if (n >= chunk_steps) {
+            if (!g_chunk[cur]) g_chunk[cur] = capture(cur, chunk_steps);
+            HIPCHK(hipGraphLaunch(g_chunk[cur], stream));
+            if (chunk_steps & 1) cur ^= 1;
+            steps += chunk_steps;
+            return chunk_steps;
         }
-        check_err();
+        if (!g_one[cur]) g_one[cur] = capture(cur, 1);
+        HIPCHK(hipGraphLaunch(g_one[cur], stream));
+        cur ^= 1;
+        steps += 1;
+        return 1;
     }
 };
 
 namespace {
 
 // Derived constants, main.cpp:345-907 expressions in double.
-void derive(xmd_engine &E, const xmd_topology *t) {
+void derive(Part &E, const xmd_topology *t) {
     const xmd_params &p = E.P;
     Derived &G = E.G;
     G.dt = p.dt;
@@ -445,7 +444,7 @@ void derive(xmd_engine &E, const xmd_topology *t) {
     E.pid_split = p.pid_enabled != 0;
 }
 
-void build(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
+void build(Part &E, const xmd_params *p, const xmd_topology *t) {
     if (!p || !t) throw Fail{XMD_ERR_ARG, "null params or topology"};
     E.P = *p;
     if (!(p->dt > 0)) throw Fail{XMD_ERR_ARG, "dt must be positive"};
@@ -826,56 +825,25 @@ xmd_status run_guarded(std::string &err, const std::function<void()> &fn) {
 }
 }  // namespace
 
-extern "C" {
-
-const char *xmd_version(void) { return "xmd-hip 0.1.0 gfx950"; }
-
-int64_t xmd_rng_first_zero_draw(uint64_t state, int64_t n_draws, uint32_t slot_mask) {
-    for (int64_t k = 0; k < n_draws; ++k) {
-        if (((slot_mask >> (k & 3)) & 1u) && (xmd_dev::rng_out(state) >> 32) == 0) return k;
-        state = xmd_dev::rng_step(state);
-    }
-    return -1;
-}
-
-const char *xmd_last_error(const xmd_engine *e) { return e ? e->err.c_str() : g_create_err.c_str(); }
-
-xmd_status xmd_engine_create(const xmd_params *p, const xmd_topology *t, xmd_engine **out) {
-    if (!out) {
-        g_create_err = "null output pointer";
-        return XMD_ERR_ARG;
-    }
-    *out = nullptr;
-    xmd_engine *e = new xmd_engine();
-    xmd_status st = run_guarded(g_create_err, [&]() { build(*e, p, t); });
-    if (st != XMD_OK) {
-        delete e;
-        return st;
-    }
-    *out = e;
-    return XMD_OK;
-}
-
-void xmd_engine_destroy(xmd_engine *e) { delete e; }
-
-void *xmd_stream(xmd_engine *e) { return e ? (void *)e->stream : nullptr; }
+namespace {
+void *xmd_stream(Part *e) { return e ? (void *)e->stream : nullptr; }
 
 // Positions of slots [s0, s0 + nb) from / to host SoA (x | y | z); the
 // device keeps them AoS, 4 doubles per slot.
-static void put_positions(xmd_engine *e, int s0, int nb, const double *r) {
+static void put_positions(Part *e, int s0, int nb, const double *r) {
     std::vector<double> a(4 * (size_t)nb, 0.0);
     for (int i = 0; i < nb; ++i)
         for (int d = 0; d < 3; ++d) a[4 * (size_t)i + d] = r[(size_t)d * nb + i];
     HIPCHK(hipMemcpy(e->D.pos[e->cur] + 4 * (size_t)s0, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
 }
-static void get_positions(xmd_engine *e, int s0, int nb, double *r) {
+static void get_positions(Part *e, int s0, int nb, double *r) {
     std::vector<double> a(4 * (size_t)nb);
     HIPCHK(hipMemcpy(a.data(), e->D.pos[e->cur] + 4 * (size_t)s0, a.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < nb; ++i)
         for (int d = 0; d < 3; ++d) r[(size_t)d * nb + i] = a[4 * (size_t)i + d];
 }
 
-xmd_status xmd_set_state(xmd_engine *e, int32_t rep, const double *y, uint64_t rng, double t) {
+static xmd_status part_set_state(Part *e, int32_t rep, const double *y, uint64_t rng, double t) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep || !y) throw Fail{XMD_ERR_ARG, "bad replica or null state"};
@@ -894,7 +862,7 @@ xmd_status xmd_set_state(xmd_engine *e, int32_t rep, const double *y, uint64_t r
     });
 }
 
-xmd_status xmd_get_state(xmd_engine *e, int32_t rep, double *y, double *v, uint64_t *rng, double *t) {
+static xmd_status part_get_state(Part *e, int32_t rep, double *y, double *v, uint64_t *rng, double *t) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
@@ -920,7 +888,7 @@ xmd_status xmd_get_state(xmd_engine *e, int32_t rep, double *y, double *v, uint6
     });
 }
 
-xmd_status xmd_set_positions(xmd_engine *e, int32_t rep, const double *r) {
+static xmd_status part_set_positions(Part *e, int32_t rep, const double *r) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep || !r) throw Fail{XMD_ERR_ARG, "bad replica or null positions"};
@@ -931,7 +899,7 @@ xmd_status xmd_set_positions(xmd_engine *e, int32_t rep, const double *r) {
     });
 }
 
-xmd_status xmd_rebuild_lists(xmd_engine *e) {
+static xmd_status part_rebuild_lists(Part *e) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         std::fill(e->pending_rb.begin(), e->pending_rb.end(), 1);
@@ -939,7 +907,7 @@ xmd_status xmd_rebuild_lists(xmd_engine *e) {
     });
 }
 
-xmd_status xmd_get_list(xmd_engine *e, int32_t rep, int32_t id, int32_t *i1, int32_t *i2, int64_t cap, int64_t *nout) {
+static xmd_status part_get_list(Part *e, int32_t rep, int32_t id, int32_t *i1, int32_t *i2, int64_t cap, int64_t *nout) {
     if (!e || !nout) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
@@ -1005,7 +973,7 @@ xmd_status xmd_get_list(xmd_engine *e, int32_t rep, int32_t id, int32_t *i1, int
     });
 }
 
-xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double *V_terms) {
+static xmd_status part_eval_forces(Part *e, uint32_t term_mask, double *F, double *V_terms) {
     if (!e || !F) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         e->ensure_lists();
@@ -1041,7 +1009,7 @@ xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double 
     });
 }
 
-xmd_status xmd_qa_update(xmd_engine *e) {
+static xmd_status part_qa_update(Part *e) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         e->ensure_lists();
@@ -1053,7 +1021,7 @@ xmd_status xmd_qa_update(xmd_engine *e) {
     });
 }
 
-xmd_status xmd_get_qa_state(xmd_engine *e, int32_t rep, xmd_qa_contact *c, int64_t cap, int64_t *nout, int8_t *sync) {
+static xmd_status part_get_qa_state(Part *e, int32_t rep, xmd_qa_contact *c, int64_t cap, int64_t *nout, int8_t *sync) {
     if (!e || !nout) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
@@ -1089,7 +1057,7 @@ xmd_status xmd_get_qa_state(xmd_engine *e, int32_t rep, xmd_qa_contact *c, int64
     });
 }
 
-xmd_status xmd_set_qa_state(xmd_engine *e, int32_t rep, const xmd_qa_contact *c, int64_t nc, const int8_t *sync) {
+static xmd_status part_set_qa_state(Part *e, int32_t rep, const xmd_qa_contact *c, int64_t nc, const int8_t *sync) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
@@ -1139,12 +1107,8 @@ xmd_status xmd_set_qa_state(xmd_engine *e, int32_t rep, const xmd_qa_contact *c,
     });
 }
 
-xmd_status xmd_step(xmd_engine *e, int64_t nsteps) {
-    if (!e || nsteps < 0) return XMD_ERR_ARG;
-    return run_guarded(e->err, [&]() { e->run_steps(nsteps); });
-}
 
-xmd_status xmd_get_energy(xmd_engine *e, double *V) {
+static xmd_status part_get_energy(Part *e, double *V) {
     if (!e || !V) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         auto Vb = e->get(e->D.Vbead, e->nslot);
@@ -1156,7 +1120,7 @@ xmd_status xmd_get_energy(xmd_engine *e, double *V) {
     });
 }
 
-xmd_status xmd_get_counters(xmd_engine *e, int64_t *out) {
+static xmd_status part_get_counters(Part *e, int64_t *out) {
     if (!e || !out) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         auto rb = e->get(e->D.n_rebuild, e->nrep);
@@ -1178,7 +1142,7 @@ xmd_status xmd_get_counters(xmd_engine *e, int64_t *out) {
     });
 }
 
-xmd_status xmd_profile(xmd_engine *e, int64_t nsteps, char *names, int64_t names_cap, int64_t *counts,
+static xmd_status part_profile(Part *e, int64_t nsteps, char *names, int64_t names_cap, int64_t *counts,
                        double *total_ms, int32_t cap, int32_t *n_kernels) {
     if (!e || nsteps < 0 || !n_kernels) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
@@ -1220,7 +1184,7 @@ xmd_status xmd_profile(xmd_engine *e, int64_t nsteps, char *names, int64_t names
     });
 }
 
-xmd_status xmd_rand_normals(xmd_engine *e, uint64_t state, int64_t nb, float *out, uint64_t *state_out) {
+static xmd_status part_rand_normals(Part *e, uint64_t state, int64_t nb, float *out, uint64_t *state_out) {
     if (!e || nb < 0 || (nb > 0 && !out)) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         if (nb > (1LL << 28)) throw Fail{XMD_ERR_ARG, "too many beads"};
@@ -1239,6 +1203,315 @@ xmd_status xmd_rand_normals(xmd_engine *e, uint64_t state, int64_t nb, float *ou
             *state_out = apply(mpow(J, 4ull * (uint64_t)nb), state);
         }
     });
+}
+
+}  // namespace
+
+// ================================================================ parts ===
+// An engine advances its replicas as one or more parts (contiguous replica
+// ranges of about equal bead count), each with its own device state, HIP
+// stream and captured step graphs.  With two parts, one part's
+// latency-bound kernels (QA sift, integrator, list rebuild) run while the
+// other's force kernel holds the chip: xmd_step launches their graphs
+// interleaved and waits once.  Results per replica do not depend on the
+// split (kernels never mix replicas).
+struct xmd_engine {
+    std::vector<Part *> parts;
+    std::vector<int> rep_part, rep_local, part_rep0, part_bead0;
+    int nrep = 0, nbeads = 0;
+    std::string err;
+    ~xmd_engine() {
+        for (Part *q : parts) delete q;
+    }
+};
+
+namespace {
+
+// Runs a part-level call and copies the part's error message.
+xmd_status on_part(xmd_engine *e, Part *q, xmd_status st) {
+    if (st != XMD_OK) e->err = q->err;
+    return st;
+}
+
+// The sub-topology of replicas [r0, r1) with indices rebased to its first
+// bead; `store` keeps the arrays it points into alive.
+struct SubTopo {
+    xmd_topology t{};
+    std::vector<int32_t> rep_off, chain_off, i1, i2;
+    std::vector<double> r0, dist;
+};
+void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
+    const int b0 = t->replica_offset[r0], b1 = t->replica_offset[r1], nb = b1 - b0;
+    for (int r = r0; r <= r1; ++r) o.rep_off.push_back(t->replica_offset[r] - b0);
+    for (int c = 0; c < t->n_chains; ++c)
+        if (t->chain_offset[c] >= b0 && t->chain_offset[c] < b1) o.chain_off.push_back(t->chain_offset[c] - b0);
+    o.chain_off.push_back(nb);
+    o.r0.resize(3 * (size_t)nb);
+    for (int d = 0; d < 3; ++d)
+        std::memcpy(o.r0.data() + (size_t)d * nb, t->r0 + (size_t)d * t->n_beads + b0, nb * sizeof(double));
+    for (int k = 0; k < t->n_native; ++k)
+        if (t->nat_i1[k] >= b0 && t->nat_i1[k] < b1) {
+            if (t->nat_i2[k] < b0 || t->nat_i2[k] >= b1) throw Fail{XMD_ERR_ARG, "native contact spans replicas"};
+            o.i1.push_back(t->nat_i1[k] - b0);
+            o.i2.push_back(t->nat_i2[k] - b0);
+            o.dist.push_back(t->nat_dist[k]);
+        }
+    auto at = [&](const double *a) { return a ? a + b0 : nullptr; };
+    o.t.n_replicas = r1 - r0;
+    o.t.n_beads = nb;
+    o.t.n_chains = (int32_t)o.chain_off.size() - 1;
+    o.t.n_native = (int32_t)o.i1.size();
+    o.t.replica_offset = o.rep_off.data();
+    o.t.chain_offset = o.chain_off.data();
+    o.t.atype = t->atype + b0;
+    o.t.r0 = o.r0.data();
+    o.t.box = t->box + 3 * (size_t)r0;
+    o.t.temperature = t->temperature + r0;
+    o.t.rng_state = t->rng_state + r0;
+    o.t.nat_i1 = o.i1.data();
+    o.t.nat_i2 = o.i2.data();
+    o.t.nat_dist = o.dist.data();
+    o.t.tether_len = at(t->tether_len);
+    o.t.nat_theta = at(t->nat_theta);
+    o.t.nat_phi = at(t->nat_phi);
+    o.t.nat_chir = at(t->nat_chir);
+    o.t.nat_chir_factor = at(t->nat_chir_factor);
+}
+
+void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
+    if (!p || !t) throw Fail{XMD_ERR_ARG, "null params or topology"};
+    if (t->n_replicas <= 0 || t->n_beads <= 0 || !t->replica_offset || !t->chain_offset || !t->r0)
+        throw Fail{XMD_ERR_ARG, "empty topology"};
+    if (t->replica_offset[0] != 0 || t->replica_offset[t->n_replicas] != t->n_beads)
+        throw Fail{XMD_ERR_ARG, "replica_offset must span [0, n_beads]"};
+    if (p->stream_groups < 0) throw Fail{XMD_ERR_ARG, "stream_groups must be >= 0"};
+    const int R = t->n_replicas;
+    const int G = std::min(R, p->stream_groups > 0 ? p->stream_groups : (R >= 2 ? 2 : 1));
+    // cut points: the first replica starting at or after g/G of the beads
+    std::vector<int> cut{0};
+    for (int g = 1; g < G; ++g) {
+        long long target = (long long)t->n_beads * g / G;
+        int r = cut.back() + 1;
+        while (r < R - (G - g) && t->replica_offset[r] < target) ++r;
+        cut.push_back(r);
+    }
+    cut.push_back(R);
+    E.nrep = R;
+    E.nbeads = t->n_beads;
+    E.rep_part.resize(R);
+    E.rep_local.resize(R);
+    for (int g = 0; g < G; ++g) {
+        SubTopo st;
+        sub_topology(t, cut[g], cut[g + 1], st);
+        Part *q = new Part();
+        E.parts.push_back(q);
+        build(*q, p, &st.t);
+        E.part_rep0.push_back(cut[g]);
+        E.part_bead0.push_back(t->replica_offset[cut[g]]);
+        for (int r = cut[g]; r < cut[g + 1]; ++r) {
+            E.rep_part[r] = g;
+            E.rep_local[r] = r - cut[g];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *xmd_version(void) { return "xmd-hip 0.1.0 gfx950"; }
+
+int64_t xmd_rng_first_zero_draw(uint64_t state, int64_t n_draws, uint32_t slot_mask) {
+    for (int64_t k = 0; k < n_draws; ++k) {
+        if (((slot_mask >> (k & 3)) & 1u) && (xmd_dev::rng_out(state) >> 32) == 0) return k;
+        state = xmd_dev::rng_step(state);
+    }
+    return -1;
+}
+
+const char *xmd_last_error(const xmd_engine *e) { return e ? e->err.c_str() : g_create_err.c_str(); }
+
+xmd_status xmd_engine_create(const xmd_params *p, const xmd_topology *t, xmd_engine **out) {
+    if (!out) {
+        g_create_err = "null output pointer";
+        return XMD_ERR_ARG;
+    }
+    *out = nullptr;
+    xmd_engine *e = new xmd_engine();
+    xmd_status st = run_guarded(g_create_err, [&]() { create(*e, p, t); });
+    if (st != XMD_OK) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return XMD_OK;
+}
+
+void xmd_engine_destroy(xmd_engine *e) { delete e; }
+
+void *xmd_stream(xmd_engine *e) { return e ? (void *)e->parts[0]->stream : nullptr; }
+
+#define XMD_REPLICA_CALL(call)                                                                                      \
+    if (!e) return XMD_ERR_ARG;                                                                                     \
+    if (rep < 0 || rep >= e->nrep) {                                                                                \
+        e->err = "bad replica";                                                                                     \
+        return XMD_ERR_ARG;                                                                                         \
+    }                                                                                                               \
+    Part *q = e->parts[e->rep_part[rep]];                                                                           \
+    const int lr = e->rep_local[rep];                                                                               \
+    (void)lr;                                                                                                       \
+    return on_part(e, q, call);
+
+xmd_status xmd_set_state(xmd_engine *e, int32_t rep, const double *y, uint64_t rng, double t) {
+    XMD_REPLICA_CALL(part_set_state(q, lr, y, rng, t))
+}
+xmd_status xmd_get_state(xmd_engine *e, int32_t rep, double *y, double *v, uint64_t *rng, double *t) {
+    XMD_REPLICA_CALL(part_get_state(q, lr, y, v, rng, t))
+}
+xmd_status xmd_set_positions(xmd_engine *e, int32_t rep, const double *r) {
+    XMD_REPLICA_CALL(part_set_positions(q, lr, r))
+}
+xmd_status xmd_get_list(xmd_engine *e, int32_t rep, int32_t id, int32_t *i1, int32_t *i2, int64_t cap, int64_t *nout) {
+    XMD_REPLICA_CALL(part_get_list(q, lr, id, i1, i2, cap, nout))
+}
+xmd_status xmd_get_qa_state(xmd_engine *e, int32_t rep, xmd_qa_contact *c, int64_t cap, int64_t *nout, int8_t *sync) {
+    XMD_REPLICA_CALL(part_get_qa_state(q, lr, c, cap, nout, sync))
+}
+xmd_status xmd_set_qa_state(xmd_engine *e, int32_t rep, const xmd_qa_contact *c, int64_t nc, const int8_t *sync) {
+    XMD_REPLICA_CALL(part_set_qa_state(q, lr, c, nc, sync))
+}
+
+xmd_status xmd_rebuild_lists(xmd_engine *e) {
+    if (!e) return XMD_ERR_ARG;
+    for (Part *q : e->parts) {
+        xmd_status st = on_part(e, q, part_rebuild_lists(q));
+        if (st != XMD_OK) return st;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_qa_update(xmd_engine *e) {
+    if (!e) return XMD_ERR_ARG;
+    for (Part *q : e->parts) {
+        xmd_status st = on_part(e, q, part_qa_update(q));
+        if (st != XMD_OK) return st;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_eval_forces(xmd_engine *e, uint32_t term_mask, double *F, double *V_terms) {
+    if (!e || !F) return XMD_ERR_ARG;
+    for (size_t g = 0; g < e->parts.size(); ++g) {
+        Part *q = e->parts[g];
+        std::vector<double> Fp(3 * (size_t)q->nbeads), Vp((size_t)q->nrep * XMD_NUM_TERMS);
+        xmd_status st = on_part(e, q, part_eval_forces(q, term_mask, Fp.data(), V_terms ? Vp.data() : nullptr));
+        if (st != XMD_OK) return st;
+        for (int d = 0; d < 3; ++d)
+            std::memcpy(F + (size_t)d * e->nbeads + e->part_bead0[g], Fp.data() + (size_t)d * q->nbeads,
+                        q->nbeads * sizeof(double));
+        if (V_terms) std::memcpy(V_terms + (size_t)e->part_rep0[g] * XMD_NUM_TERMS, Vp.data(), Vp.size() * sizeof(double));
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_step(xmd_engine *e, int64_t nsteps) {
+    if (!e || nsteps < 0) return XMD_ERR_ARG;
+    for (Part *q : e->parts) {
+        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { q->ensure_lists(); }));
+        if (st != XMD_OK) return st;
+    }
+    // interleave the parts' graph launches so their streams run side by side
+    std::vector<int64_t> left(e->parts.size(), nsteps);
+    bool more = nsteps > 0;
+    while (more) {
+        more = false;
+        for (size_t g = 0; g < e->parts.size(); ++g) {
+            Part *q = e->parts[g];
+            if (left[g] <= 0) continue;
+            xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { left[g] -= q->launch_next(left[g]); }));
+            if (st != XMD_OK) return st;
+            more |= left[g] > 0;
+        }
+    }
+    for (Part *q : e->parts) {
+        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { q->check_err(); }));
+        if (st != XMD_OK) return st;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_get_energy(xmd_engine *e, double *V) {
+    if (!e || !V) return XMD_ERR_ARG;
+    for (size_t g = 0; g < e->parts.size(); ++g) {
+        xmd_status st = on_part(e, e->parts[g], part_get_energy(e->parts[g], V + e->part_rep0[g]));
+        if (st != XMD_OK) return st;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_get_counters(xmd_engine *e, int64_t *out) {
+    if (!e || !out) return XMD_ERR_ARG;
+    for (int k = 0; k < 5; ++k) out[k] = 0;
+    for (Part *q : e->parts) {
+        int64_t c[5];
+        xmd_status st = on_part(e, q, part_get_counters(q, c));
+        if (st != XMD_OK) return st;
+        out[0] = c[0];
+        for (int k = 1; k < 5; ++k) out[k] += c[k];
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_profile(xmd_engine *e, int64_t nsteps, char *names, int64_t names_cap, int64_t *counts,
+                       double *total_ms, int32_t cap, int32_t *n_kernels) {
+    if (!e || nsteps < 0 || !n_kernels) return XMD_ERR_ARG;
+    // each part in turn, eagerly; kernels merged by name
+    std::vector<std::string> nm;
+    std::vector<int64_t> cnt;
+    std::vector<double> ms;
+    for (Part *q : e->parts) {
+        std::vector<char> nb(8192);
+        std::vector<int64_t> c(256);
+        std::vector<double> t(256);
+        int32_t nk = 0;
+        xmd_status st = on_part(e, q, part_profile(q, nsteps, nb.data(), (int64_t)nb.size(), c.data(), t.data(), 256, &nk));
+        if (st != XMD_OK) return st;
+        std::string all(nb.data());
+        size_t pos = 0;
+        for (int k = 0; k < nk && k < 256; ++k) {
+            size_t end = all.find('\n', pos);
+            std::string name = all.substr(pos, end - pos);
+            pos = end + 1;
+            size_t i = std::find(nm.begin(), nm.end(), name) - nm.begin();
+            if (i == nm.size()) {
+                nm.push_back(name);
+                cnt.push_back(0);
+                ms.push_back(0.0);
+            }
+            cnt[i] += c[k];
+            ms[i] += t[k];
+        }
+    }
+    *n_kernels = (int32_t)nm.size();
+    std::string all;
+    for (size_t k = 0; k < nm.size(); ++k) {
+        if ((int32_t)k < cap) {
+            if (counts) counts[k] = cnt[k];
+            if (total_ms) total_ms[k] = ms[k];
+        }
+        all += nm[k] + "\n";
+    }
+    if (names && names_cap > 0) {
+        size_t m = std::min<size_t>(all.size(), (size_t)names_cap - 1);
+        std::memcpy(names, all.data(), m);
+        names[m] = 0;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_rand_normals(xmd_engine *e, uint64_t state, int64_t nb, float *out, uint64_t *state_out) {
+    if (!e) return XMD_ERR_ARG;
+    return on_part(e, e->parts[0], part_rand_normals(e->parts[0], state, nb, out, state_out));
 }
 
 }  // extern "C"
