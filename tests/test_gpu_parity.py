@@ -186,6 +186,36 @@ def test_batched_replicas_match_single():
         assert np.abs(y - yo).max() < 1e-9
 
 
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_MIXED])
+def test_stream_groups_do_not_change_results(prec):
+    """Replicas stepped as one part or as two concurrent stream groups give
+    bit-identical forces, energies, lists, QA state and trajectories: no
+    kernel mixes replicas."""
+    system = synth.config_system(2, n_batch=3, chains=6, horizon_steps=80)
+    engs = []
+    for groups in (1, 2):
+        params = synth.config_params(2, precision=prec)
+        params.stream_groups = groups
+        engs.append(Engine(params, system))
+    a, b = engs
+    Fa, Va = a.eval_forces()
+    Fb, Vb = b.eval_forces()
+    assert np.array_equal(Fa, Fb) and np.array_equal(Va, Vb)
+    a.step(40)
+    b.step(40)
+    assert a.counters()["rebuilds"] == b.counters()["rebuilds"]
+    for k in range(system.n_replicas):
+        ya, _, ra, ta = a.get_state(k)
+        yb, _, rb, tb = b.get_state(k)
+        assert ra == rb and ta == tb and np.array_equal(ya, yb)
+        ca, sa = a.get_qa_state(k)
+        cb, sb = b.get_qa_state(k)
+        assert np.array_equal(ca, cb) and np.array_equal(sa, sb)
+        for name in _abi.LISTS:
+            la, lb = a.get_list(k, name), b.get_list(k, name)
+            assert np.array_equal(la[0], lb[0]) and np.array_equal(la[1], lb[1]), name
+
+
 @pytest.mark.parametrize("kind,d", [("ss", 6.0), ("ss", 7.0), ("bb", 4.8)])
 def test_qa_timeline_engine(kind, d):
     import qa_scenarios as Q

@@ -174,7 +174,11 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
                     const int i1 = owner ? s : j, i2 = owner ? j : s;
                     const int ty = 20 * D.atype[i1] + D.atype[i2];
                     v3<T> out[6];
-                    T v = pid_bundle<T>(P, (T)D.ss_depth[ty], (T)D.ss_rmin[ty], (T)D.ss_rmax[ty],
+                    const bool f32 = sizeof(T) == 4;  // float copies of the ss tables in the FP32 form
+                    const T ssd = f32 ? (T)D.ss_depth_f[ty] : (T)D.ss_depth[ty];
+                    const T ssn = f32 ? (T)D.ss_rmin_f[ty] : (T)D.ss_rmin[ty];
+                    const T ssx = f32 ? (T)D.ss_rmax_f[ty] : (T)D.ss_rmax[ty];
+                    T v = pid_bundle<T>(P, ssd, ssn, ssx,
                                         pid_side<T>(ri - rsp, rsn - ri, rsn - rsp),
                                         pid_side<T>(rj - rjp, rjn - rj, rjn - rjp), dd, out);
                     add3(Fx, Fy, Fz, out[1]);
