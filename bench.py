@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the 512-chain and 100k-bead workloads")
+    ap.add_argument("--extra-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy and a scale kernel of MIB MiB each "
                          "(known HBM bytes, to calibrate FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -127,7 +129,10 @@ def main():
 
     # ---- batched mixed precision (the headline form)
     eng = Engine(p_mixed, batch)
-    dt_mixed = timed_steps(eng, args.warmup, args.steps)
+    eng.step(args.warmup)
+    lanes, tune_ms = eng.autotune(reps=10)
+    log(f"autotune: lanes per bead {lanes} (ms per step for 4/8/16: {[round(x, 4) for x in tune_ms]})")
+    dt_mixed = timed_steps(eng, 0, args.steps)
     value = batch.n_beads * args.steps / dt_mixed
     log(f"mixed batched: {args.steps} steps in {dt_mixed:.3f}s -> {value:.3e} bead-steps/s")
     counters = eng.counters()
@@ -181,6 +186,23 @@ def main():
         rel_rms = float(np.sqrt(((Fm - F64) ** 2).sum() / (F64 ** 2).sum()))
         log(f"mixed vs fp64 force rel-RMS: {rel_rms:.3e}")
 
+    # ---- secondary workloads (mixed precision, autotuned): 512 single chains
+    # of 30..400 beads in open boxes, and one 100k-bead periodic box
+    extra = {}
+    if not args.no_extra:
+        for key, cid, kw in (("c3_512_single_chains", 3, {}), ("c4_box_100k", 4, {})):
+            t1 = time.time()
+            sysx = synth.config_system(cid, horizon_steps=args.extra_steps + 150, **kw)
+            ex = Engine(synth.config_params(cid, precision=_abi.PREC_MIXED), sysx)
+            ex.step(100)
+            lx, _ = ex.autotune(reps=5)
+            dtx = timed_steps(ex, 0, args.extra_steps)
+            extra[key] = {"value": round(sysx.n_beads * args.extra_steps / dtx, 1), "unit": "bead-steps/s",
+                          "beads": int(sysx.n_beads), "replicas": int(sysx.n_replicas), "steps": args.extra_steps,
+                          "ms_per_step": round(1e3 * dtx / args.extra_steps, 4), "lanes_per_bead": lx}
+            log(f"{key}: {extra[key]} (setup {time.time() - t1:.1f}s)")
+            ex.close()
+
     # ---- CPU baseline: the oracle in the reference's OpenMP structure
     cpu = None
     if not args.no_cpu:
@@ -229,6 +251,8 @@ def main():
         "rebuilds": counters["rebuilds"],
         "nl_entries": counters["nl_entries"],
         "qa_contacts": counters["qa_contacts"],
+        "autotune": {"lanes_per_bead": lanes, "ms_per_step_4_8_16": [round(x, 4) for x in tune_ms]},
+        "secondary": extra,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

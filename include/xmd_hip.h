@@ -150,7 +150,9 @@ typedef struct xmd_params {
     /* per aa: back, side_all, side_polar, side_hydrophobic (sync_data order) */
     int8_t aa_limits[XMD_NUM_AA][4];
     int8_t aa_polarization[XMD_NUM_AA]; /* 0 none, 1 polar, 2 hydrophobic */
-    int32_t _pad8;
+    int32_t force_lanes;        /* lanes per bead of the force kernel: 0 = 8,
+                                   or 4, 8, 16 (xmd_autotune picks one by
+                                   timing); no counterpart in freeform_params */
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and
@@ -273,6 +275,13 @@ xmd_status xmd_profile(xmd_engine *eng, int64_t nsteps, char *names, int64_t nam
 
 /* Stream used by every launch of this engine (hipStream_t as void*). */
 void *xmd_stream(xmd_engine *eng);
+
+/* Autotuner (no counterpart in the reference): times `reps` launches of the
+ * force kernel with 4, 8 and 16 lanes per bead on the current state, keeps
+ * the fastest for every later step and returns it in *lpb_out and the three
+ * times (ms per step, summed over stream groups) in ms_out[3] (either may be
+ * NULL).  The simulation state is not changed. */
+xmd_status xmd_autotune(xmd_engine *eng, int32_t reps, int32_t *lpb_out, double *ms_out);
 #ifdef __cplusplus
 }
 #endif

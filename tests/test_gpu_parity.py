@@ -216,6 +216,50 @@ def test_stream_groups_do_not_change_results(prec):
             assert np.array_equal(la[0], lb[0]) and np.array_equal(la[1], lb[1]), name
 
 
+@pytest.mark.parametrize("lanes", [4, 16])
+def test_force_lanes_variants(c2_small, lanes):
+    """The 4- and 16-lanes-per-bead variants of the force kernel (the
+    autotuner's other choices): per-term forces and energies against the
+    oracle on an evolved, perturbed periodic conformation."""
+    params, system = c2_small
+    params = _abi.XmdParams.from_buffer_copy(params)
+    params.force_lanes = lanes
+    eng, ora = Engine(params, system), Oracle(params, system)
+    ora.step(300)
+    y, _, rng, t = ora.get_state(0)
+    cont, sync = ora.get_qa_state(0)
+    eng.set_state(0, y, rng, t)
+    eng.set_qa_state(0, cont, sync)
+    ora.set_state(0, y, rng, t)
+    r = y[0] + np.random.default_rng(6).normal(scale=0.3 * 0.2, size=y[0].shape)
+    eng.set_positions(0, r)
+    ora.set_positions(0, r)
+    compare_terms(eng, ora)
+
+
+def test_autotune_keeps_state():
+    """The autotuner times force launches only: the state is unchanged and
+    the trajectory stays in parity with the oracle afterwards."""
+    system = synth.config_system(2, n_batch=2, chains=6, horizon_steps=60)
+    params = synth.config_params(2)
+    eng = Engine(params, system)
+    ora = Oracle(params, system)
+    eng.step(10)
+    ora.step(10)
+    before = [eng.get_state(k) for k in range(2)]
+    lpb, ms = eng.autotune(reps=3)
+    assert lpb in (4, 8, 16) and len(ms) == 3 and all(m > 0 for m in ms)
+    for k in range(2):
+        y, _, rng, t = eng.get_state(k)
+        assert np.array_equal(y, before[k][0]) and rng == before[k][2] and t == before[k][3]
+    eng.step(10)
+    ora.step(10)
+    for k in range(2):
+        y, _, rng, t = eng.get_state(k)
+        yo, _, rngo, to = ora.get_state(k)
+        assert rng == rngo and t == to and np.abs(y - yo).max() < 1e-9
+
+
 @pytest.mark.parametrize("kind,d", [("ss", 6.0), ("ss", 7.0), ("bb", 4.8)])
 def test_qa_timeline_engine(kind, d):
     import qa_scenarios as Q

@@ -60,7 +60,7 @@ class XmdParams(C.Structure):
         ("qa_min_cos_hr", _d), ("qa_min_cos_hh", _d), ("qa_max_cos_nr", _d),
         ("aa_limits", (_i8 * 4) * NUM_AA),
         ("aa_polarization", _i8 * NUM_AA),
-        ("_pad8", _i32),
+        ("force_lanes", _i32),
     ]
 
 

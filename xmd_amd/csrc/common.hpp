@@ -105,12 +105,15 @@ __device__ inline void st3(double *a, int n, int s, v3<double> v) {
 __device__ inline int lane_id() { return threadIdx.x & (WAVE - 1); }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by block index, and
-// each XCD has its own L2.  For a grid of nb blocks (a multiple of 8) this
-// maps block b to a position so that XCD x covers the contiguous eighth
+// each XCD has its own L2.  For a grid of nb blocks (a multiple of 8; other
+// grids keep their order) this maps block b to a position so that XCD x
+// covers the contiguous eighth
 // [x nb/8, (x+1) nb/8) of the grid: the partner gathers of one XCD then
 // stay within a few replicas, whose positions fit its L2.
 constexpr int NUM_XCD = 8;
-__device__ inline int xcd_block(int b, int nb) { return (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD; }
+__device__ inline int xcd_block(int b, int nb) {
+    return nb % NUM_XCD ? b : (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
+}
 
 // Parameters of the force kernel in its precision T, and the term mask.
 template <typename T> struct TermTab {
@@ -120,8 +123,9 @@ template <typename T> struct TermTab {
 
 // Lanes per bead of the force kernel, and its launch (forces.hip, a
 // translation unit of its own compiled with multiply-add contraction).
+// Lanes per bead: 4, 8 (default) or 16, chosen by the autotuner.
 constexpr int FORCE_LPB = 8;
-void launch_force(bool mixed, bool eval, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
+void launch_force(bool mixed, bool eval, int lpb, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
                   const TermTab<float> &Tf, const TermTab<double> &Td, int p);
 
 }  // namespace xmd_k

@@ -160,6 +160,44 @@ struct Part {
     uint32_t mask = 0;
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr}, g_one[2] = {nullptr, nullptr};
     int chunk_steps = 16;
+    int lpb = FORCE_LPB;  // lanes per bead of the force kernel
+    // Mean duration (ms) of `reps` force-kernel launches with `l` lanes per
+    // bead on the current state.  The kernel only writes the force buffers,
+    // which every step recomputes, so the simulation state is untouched.
+    double time_force(int l, int reps) {
+        ensure_lists();
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        const bool mixed = P.precision == XMD_PREC_MIXED;
+        launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur);  // warm-up
+        HIPCHK(hipEventRecord(a, stream));
+        for (int k = 0; k < reps; ++k) launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur);
+        HIPCHK(hipEventRecord(b, stream));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        HIPCHK(hipEventDestroy(a));
+        HIPCHK(hipEventDestroy(b));
+        HIPCHK(hipGetLastError());
+        return ms / reps;
+    }
+    // Use `l` lanes per bead from now on (graphs are captured again).
+    void set_lpb(int l) {
+        if (l == lpb) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto &g : g_chunk)
+            if (g) {
+                HIPCHK(hipGraphExecDestroy(g));
+                g = nullptr;
+            }
+        for (auto &g : g_one)
+            if (g) {
+                HIPCHK(hipGraphExecDestroy(g));
+                g = nullptr;
+            }
+        lpb = l;
+    }
     int64_t steps = 0;
     std::vector<char> pending_rb;
     std::vector<int> fp_base_h, fp_cap_h, ct_base_h, ct_cap_h;
@@ -275,7 +313,7 @@ struct Part {
             launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
         }
         launch("k_force", [&] {
-            launch_force(P.precision == XMD_PREC_MIXED, false, nslot, stream, D, G, TTf, TTd, p);
+            launch_force(P.precision == XMD_PREC_MIXED, false, lpb, nslot, stream, D, G, TTf, TTd, p);
         });
         launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
         launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1); });
@@ -449,6 +487,9 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
     E.P = *p;
     if (!(p->dt > 0)) throw Fail{XMD_ERR_ARG, "dt must be positive"};
     if (p->precision != XMD_PREC_FP64 && p->precision != XMD_PREC_MIXED) throw Fail{XMD_ERR_ARG, "bad precision"};
+    if (p->force_lanes != 0 && p->force_lanes != 4 && p->force_lanes != 8 && p->force_lanes != 16)
+        throw Fail{XMD_ERR_ARG, "force_lanes must be 0, 4, 8 or 16"};
+    E.lpb = p->force_lanes ? p->force_lanes : FORCE_LPB;
     if (t->n_replicas <= 0 || t->n_beads <= 0) throw Fail{XMD_ERR_ARG, "empty topology"};
     if (t->replica_offset[0] != 0 || t->replica_offset[t->n_replicas] != t->n_beads)
         throw Fail{XMD_ERR_ARG, "replica_offset must span [0, n_beads]"};
@@ -986,7 +1027,7 @@ static xmd_status part_eval_forces(Part *e, uint32_t term_mask, double *F, doubl
         TermTab<double> Td = e->TTd;
         Tf.mask = m;
         Td.mask = m;
-        launch_force(e->P.precision == XMD_PREC_MIXED, true, e->nslot, e->stream, e->D, e->G, Tf, Td, p);
+        launch_force(e->P.precision == XMD_PREC_MIXED, true, e->lpb, e->nslot, e->stream, e->D, e->G, Tf, Td, p);
         k_gather_force<<<nb, BLK, 0, e->stream>>>(e->D);
         HIPCHK(hipGetLastError());
         int n = e->nslot;
@@ -1506,6 +1547,31 @@ xmd_status xmd_profile(xmd_engine *e, int64_t nsteps, char *names, int64_t names
         std::memcpy(names, all.data(), m);
         names[m] = 0;
     }
+    return XMD_OK;
+}
+
+xmd_status xmd_autotune(xmd_engine *e, int32_t reps, int32_t *lpb_out, double *ms_out) {
+    if (!e || reps <= 0) return XMD_ERR_ARG;
+    static const int cand[3] = {4, 8, 16};
+    double best = 0.0;
+    int pick = FORCE_LPB;
+    for (int c = 0; c < 3; ++c) {
+        double ms = 0.0;
+        for (Part *q : e->parts) {
+            xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { ms += q->time_force(cand[c], reps); }));
+            if (st != XMD_OK) return st;
+        }
+        if (ms_out) ms_out[c] = ms;
+        if (c == 0 || ms < best) {
+            best = ms;
+            pick = cand[c];
+        }
+    }
+    for (Part *q : e->parts) {
+        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { q->set_lpb(pick); }));
+        if (st != XMD_OK) return st;
+    }
+    if (lpb_out) *lpb_out = pick;
     return XMD_OK;
 }
 

@@ -62,6 +62,14 @@ class Engine(HostSystem):
     def stream(self):
         return lib().xmd_stream(self.handle)
 
+    def autotune(self, reps=10):
+        """Pick the force kernel's lanes per bead (4, 8, 16) by timing;
+        returns (chosen, [ms per step for 4, 8, 16])."""
+        lpb = C.c_int32()
+        ms = (C.c_double * 3)()
+        self._check(lib().xmd_autotune(self.handle, C.c_int32(reps), C.byref(lpb), ms))
+        return lpb.value, list(ms)
+
     def rand_normals(self, state, n):
         out = np.zeros(3 * max(n, 1), dtype=np.float32)
         so = C.c_uint64()
