@@ -115,25 +115,23 @@ XMD_HD inline float atan2_f(float y, float x) {
     if (x < 0.0f) r = 3.141592653589793f - r;
     return copysignf(r, y);
 }
-// sin and cos of h, |h| <= pi/2, by their Taylor series (truncation error
-// below 7e-10 there).
+// sin and cos of h, |h| <= pi/2: odd / even polynomials (degrees 9 and 10)
+// fitted here by iteratively reweighted least squares on Chebyshev nodes;
+// error in exact arithmetic 6e-9 (relative, sin) and 3e-10 (absolute, cos),
+// about 1 ulp evaluated in FP32.
 XMD_HD inline void sincos_half_f(float h, float &sh, float &ch) {
     const float z = h * h;
-    float ps = 1.0f / 6227020800.0f;
-    ps = ps * z - 1.0f / 39916800.0f;
-    ps = ps * z + 1.0f / 362880.0f;
-    ps = ps * z - 1.0f / 5040.0f;
-    ps = ps * z + 1.0f / 120.0f;
-    ps = ps * z - 1.0f / 6.0f;
-    sh = h + h * (ps * z);
-    float pc = -1.0f / 87178291200.0f;
-    pc = pc * z + 1.0f / 479001600.0f;
-    pc = pc * z - 1.0f / 3628800.0f;
-    pc = pc * z + 1.0f / 40320.0f;
-    pc = pc * z - 1.0f / 720.0f;
-    pc = pc * z + 1.0f / 24.0f;
+    float ps = 2.6057807644974673e-06f;
+    ps = ps * z - 0.00019809603691101074f;
+    ps = ps * z + 0.00833306647837162f;
+    ps = ps * z - 0.16666659712791443f;
+    sh = h + (h * z) * ps;
+    float pc = -2.607710882784886e-07f;
+    pc = pc * z + 2.4761886379565112e-05f;
+    pc = pc * z - 0.00138884037733078f;
+    pc = pc * z + 0.04166664183139801f;
     pc = pc * z - 0.5f;
-    ch = 1.0f + pc * z;
+    ch = 1.0f + z * pc;
 }
 
 // Orthorhombic box, xmd/include/xmd/model/box.h:36-49 (cell_inv 0 = open).
