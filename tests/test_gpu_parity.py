@@ -260,6 +260,58 @@ def test_autotune_keeps_state():
         assert rng == rngo and t == to and np.abs(y - yo).max() < 1e-9
 
 
+def degenerate_system():
+    """Chains of 1..5 beads beside two 30-bead chains, once in a periodic box
+    and once in an open box (two replicas of one engine): every boundary
+    case of the bonded windows (no prev/next, tether only, angle without
+    dihedral) and of PID (needs both neighbours) and QA (n, h = 0 at ends)."""
+    rng = np.random.default_rng(21)
+    lens = [1, 2, 3, 4, 5, 30, 30]
+    chains = [synth.random_chain(m, rng) for m in lens]
+    L = 70.0
+    placed = synth.place_chains([c[0] for c in chains], L, rng)
+    reps = [dict(chains=[(p, c[1]) for p, c in zip(placed, chains)], box_A=(L, L, L), temperature=0.35, seed=0x5EED0101),
+            dict(chains=[(p, c[1]) for p, c in zip(placed, chains)], box_A=(0, 0, 0), temperature=0.45, seed=0x5EED0102)]
+    return synth.make_system(reps)
+
+
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_MIXED])
+def test_degenerate_chains(prec):
+    system = degenerate_system()
+    params = synth.config_params(2, precision=prec)
+    synth.apply_terms(params, {"snd": True})
+    pf = synth.config_params(2)
+    synth.apply_terms(pf, {"snd": True})
+    eng, ora = Engine(params, system), Oracle(pf, system)
+    ora.rebuild_lists()
+    for k in range(2):
+        lists_equal(eng, ora, k)
+    # off the native structure (there the bonded forces are round-off)
+    r = synth.perturbed_positions(system, seed=23)
+    for k in range(2):
+        a, b = system.replica_offset[k], system.replica_offset[k + 1]
+        eng.set_positions(k, r[:, a:b])
+        ora.set_positions(k, r[:, a:b])
+    ora.rebuild_lists()
+    for k in range(2):
+        lists_equal(eng, ora, k)
+    if prec == _abi.PREC_FP64:
+        compare_terms(eng, ora, rep_count=2)
+    else:
+        F, _ = eng.eval_forces()
+        Fo, _ = ora.eval_forces()
+        assert rel_rms(F, Fo) < 1e-5
+    eng.step(30)
+    ora.step(30)
+    for k in range(2):
+        y, _, rng_e, t = eng.get_state(k)
+        yo, _, rng_o, to = ora.get_state(k)
+        assert rng_e == rng_o and t == to
+        assert np.abs(y[0] - yo[0]).max() < (1e-9 if prec == _abi.PREC_FP64 else 1e-4)
+        if prec == _abi.PREC_FP64:
+            lists_equal(eng, ora, k)
+
+
 @pytest.mark.parametrize("kind,d", [("ss", 6.0), ("ss", 7.0), ("bb", 4.8)])
 def test_qa_timeline_engine(kind, d):
     import qa_scenarios as Q
