@@ -79,6 +79,24 @@ def measured_traffic(kernel, replicas, groups):
     return int(k["hbm_bytes_per_launch"])
 
 
+def valu_issue(kernel, replicas, groups, avg_ms):
+    """VALU issue time of one launch (committed SQ_INSTS_VALU per launch x 4
+    cycles per wave64 instruction over 1024 SIMDs at 2.4 GHz) and its
+    fraction of the live launch time (a launch shares the chip with the other
+    stream group, so this is a lower bound on the issue rate)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or "valu_insts_per_launch" not in k or t.get("replicas") != replicas or t.get("groups", 1) != groups:
+        return None
+    us = k["valu_insts_per_launch"] * 4 / 1024 / 2.4e3
+    return {"issue_us": round(us, 1), "frac_of_launch": round(us / (avg_ms * 1e3), 3)}
+
+
 def copy_calibration(mib):
     """Known-traffic kernels: a D2D copy and out = 2*x, each reading and
     writing `mib` MiB once."""
@@ -104,6 +122,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the 512-chain and 100k-bead workloads")
+    ap.add_argument("--no-step-roofline", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy and a scale kernel of MIB MiB each "
@@ -152,10 +171,32 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": measured_traffic(name, args.replicas, round(per_step)),
-                    "kernel": name, "avg_ms": round(avg_ms, 5), "bytes_per_launch": int(nbytes)}
+                    "kernel": name, "avg_ms": round(avg_ms, 5), "bytes_per_launch": int(nbytes),
+                    "valu_issue": valu_issue(name, args.replicas, round(per_step), avg_ms)}
     kernels = {k: {"launches": n, "avg_us": round(1e3 * ms / n, 2), "share": round(ms / sum(v[1] for v in prof.values()), 4)}
                for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     log("kernels:", json.dumps(kernels))
+
+    # ---- whole-step algorithmic bytes per bead-step (integrator 348 B in the
+    # mixed form; per pair term P_t (8 B index pair + 4 B parameter) + 48 B
+    # per bead; bonded 88 B per bead; rebuild (36 N + 8 P_nl) over the
+    # measured rebuild interval) and the fraction of HBM peak it implies
+    step_roof = None
+    if not args.no_step_roofline:
+        P = {name: 0 for name in ("go", "pauli", "es", "pid", "nl_all")}
+        for k in range(batch.n_replicas):
+            for name in P:
+                P[name] += len(eng.get_list(k, name)[0])
+        c = eng.counters()
+        n = batch.n_beads
+        qa_pairs = c["qa_contacts"]
+        interval = c["steps"] * batch.n_replicas / max(c["rebuilds"], 1)
+        pair = sum(P[t] * 12 + n * 48 for t in ("go", "pauli", "es", "pid")) + qa_pairs * 12 + n * 48
+        per_bead = 348 + pair / n + 88 + (36 * n + 8 * P["nl_all"]) / interval / n
+        step_roof = {"bytes_per_bead_step": round(per_bead, 1), "achieved": round(per_bead * value / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(per_bead * value / 1e9 / HBM_PEAK_GBS, 4),
+                     "pairs": {**P, "qa_contacts": qa_pairs}, "rebuild_interval_steps": round(interval, 1)}
+        log(f"step roofline: {step_roof}")
 
     # ---- plain FP64 on one box, and the mixed force error against it
     fp64_value = None
@@ -219,10 +260,18 @@ def main():
         t1 = time.perf_counter()
         ora.step(nsamp, nthreads=cores)
         dtc = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        ora.step(3, nthreads=1)
+        per1 = (time.perf_counter() - t1) / 3
+        n1 = int(max(5, min(1000, 0.25 * args.cpu_seconds / max(per1, 1e-6))))
+        t1 = time.perf_counter()
+        ora.step(n1, nthreads=1)
+        dt1 = time.perf_counter() - t1
         cpu = {"value": round(single.n_beads * nsamp / dtc, 1), "unit": "bead-steps/s", "cores": cores,
                "kind": "port",
                "sample": f"one 10k-bead box (40x250, PBC), FP64, {nsamp} steps after 8 warm-up steps "
-                         f"(first list build {t_first:.2f}s excluded), OpenMP {cores} threads"}
+                         f"(first list build {t_first:.2f}s excluded), OpenMP {cores} threads",
+               "one_thread": {"value": round(single.n_beads * n1 / dt1, 1), "steps": n1}}
         log(f"cpu oracle: {cpu}")
 
     out = {
@@ -254,6 +303,7 @@ def main():
         "autotune": {"lanes_per_bead": lanes, "ms_per_step_4_8_16": [round(x, 4) for x in tune_ms]},
         "secondary": extra,
         "roofline": roofline,
+        "roofline_step": step_roof,
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
