@@ -231,9 +231,17 @@ def main():
     # of 30..400 beads in open boxes, and one 100k-bead periodic box
     extra = {}
     if not args.no_extra:
-        for key, cid, kw in (("c3_512_single_chains", 3, {}), ("c4_box_100k", 4, {})):
+        c4 = None
+        for key, cid in (("c3_512_single_chains", 3), ("c4_box_100k", 4), ("c4_batch_8x100k", 4)):
             t1 = time.time()
-            sysx = synth.config_system(cid, horizon_steps=args.extra_steps + 150, **kw)
+            if key == "c4_batch_8x100k":
+                # 8 copies of the generated box at T = 0.30 + 0.2 k/7 with their own seeds
+                seeds = synth._pick_seeds(synth.SEED_BASE + 4 + 1, 8, lambda _: c4.n_beads, args.extra_steps + 150)
+                sysx = synth.replicate(c4, [0.30 + 0.2 * k / 7 for k in range(8)], seeds)
+            else:
+                sysx = synth.config_system(cid, horizon_steps=args.extra_steps + 150)
+                if cid == 4:
+                    c4 = sysx
             ex = Engine(synth.config_params(cid, precision=_abi.PREC_MIXED), sysx)
             ex.step(100)
             lx, _ = ex.autotune(reps=5)

@@ -58,7 +58,7 @@ struct Dev {
     uint8_t *nat_act;
     // neighbour rows
     int nl_w;  // row capacity per slot (rows are fixed-width sliced ELL)
-    int *slice_start, *nl_len, *nl_half, *nl_j;
+    int *slice_start, *nl_len, *nl_j;
     uint8_t *nl_flag;
     // QA free pairs, candidates
     const int *fp_base, *fp_cap, *fp_chunk_rep;

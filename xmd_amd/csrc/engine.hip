@@ -155,7 +155,6 @@ struct Part {
     std::vector<void *> allocs;
     int cur = 0;
     std::string err;
-    bool pid_split = false;
     int nfp_chunks = 0, nct_chunks = 0, qa_grid = 0;
     uint32_t mask = 0;
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr}, g_one[2] = {nullptr, nullptr};
@@ -479,7 +478,6 @@ void derive(Part &E, const xmd_topology *t) {
     E.mask = m;
     E.TTd.mask = m;
     E.TTf.mask = m;
-    E.pid_split = p.pid_enabled != 0;
 }
 
 void build(Part &E, const xmd_params *p, const xmd_topology *t) {
@@ -734,7 +732,6 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
     int nsl = n / WAVE;
     D.slice_start = E.alloc<int>(nsl);
     D.nl_len = E.alloc<int>(n);
-    D.nl_half = E.alloc<int>(n);
     D.fp_slice_cnt = E.alloc<int>(nsl);
     D.fp_n = E.alloc<int>(n);
     D.fp_slice_start = E.alloc<int>(nsl);

@@ -757,7 +757,6 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
         }
     }
     __syncthreads();  // reached by every wave of the block: the skip test is block-uniform
-    int below = 0;
     if (active) {
         for (int k = l; k < cnt; k += WAVE) {
             const int j = lj[k];
@@ -765,7 +764,6 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
             for (int q = 0; q < cnt; ++q) r += lj[q] < j;
             D.nl_j[base + 64 * r] = j;
             D.nl_flag[base + 64 * r] = lf[k];
-            below += j < s;
         }
     }
     if (!active) {
@@ -776,12 +774,10 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
         return;
     }
     for (int o = 32; o > 0; o >>= 1) {
-        below += __shfl_xor(below, o);
         nfp += __shfl_xor(nfp, o);
     }
     if (l == 0) {
         D.nl_len[s] = cnt;
-        D.nl_half[s] = below;
         D.fp_n[s] = nfp;
         if (nfp) atomicAdd(D.fp_slice_cnt + (s >> 6), nfp);
         stp(D.orig_r, s, rs);

@@ -337,6 +337,28 @@ def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, 
     return make_system(reps)
 
 
+def replicate(system, temperatures, seeds):
+    """len(temperatures) copies of a one-replica `system` (same chains and
+    initial positions) as replicas of one batch, each with its own
+    temperature and rand_gen seed."""
+    assert system.n_replicas == 1
+    n, k = system.n_beads, len(temperatures)
+    tile = lambda a: np.concatenate([a] * k)
+    shifted = lambda a: np.concatenate([a + q * n for q in range(k)])
+    nch = len(system.chain_offset) - 1
+    return System(
+        replica_offset=np.arange(k + 1, dtype=np.int32) * n,
+        chain_offset=np.concatenate([system.chain_offset[:nch] + q * n for q in range(k)] + [[k * n]]).astype(np.int32),
+        atype=tile(system.atype),
+        r0=np.concatenate([system.r0] * k, axis=1),
+        box=np.repeat(system.box.reshape(1, 3), k, axis=0),
+        temperature=np.asarray(temperatures, dtype=np.float64),
+        rng_state=np.asarray(seeds, dtype=np.uint64),
+        nat_i1=shifted(system.nat_i1), nat_i2=shifted(system.nat_i2), nat_dist=tile(system.nat_dist),
+        tether_len=tile(system.tether_len), nat_theta=tile(system.nat_theta), nat_phi=tile(system.nat_phi),
+        nat_chir=tile(system.nat_chir), nat_chir_factor=tile(system.nat_chir_factor))
+
+
 def perturbed_positions(system, seed, sigma_A=0.3):
     """Native positions plus isotropic Gaussian displacement (C1's seeded
     conformations)."""
