@@ -276,6 +276,14 @@ xmd_status xmd_profile(xmd_engine *eng, int64_t nsteps, char *names, int64_t nam
 /* Stream used by every launch of this engine (hipStream_t as void*). */
 void *xmd_stream(xmd_engine *eng);
 
+/* Per-replica observables of the stats report (report_stats.cpp:8-41,
+ * stats/{gyration,asphericity,total_energy}.cpp, report_structure.cpp:81-85),
+ * in internal units: out[8*r + k] = t, V, K, E, Rg, W, native contacts
+ * active (|r1 - r2| < active_threshold 2^(-1/6) d_nat), native contacts.
+ * V is the energy of the last force evaluation (a step's predicted
+ * positions), as in the reference. */
+xmd_status xmd_observables(xmd_engine *eng, double active_threshold, double *out);
+
 /* Autotuner (no counterpart in the reference): times `reps` launches of the
  * force kernel with 4, 8 and 16 lanes per bead on the current state, keeps
  * the fastest for every later step and returns it in *lpb_out and the three

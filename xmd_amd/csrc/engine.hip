@@ -866,6 +866,25 @@ xmd_status run_guarded(std::string &err, const std::function<void()> &fn) {
 namespace {
 void *xmd_stream(Part *e) { return e ? (void *)e->stream : nullptr; }
 
+// Device buffer for the duration of one API call (after the stream is
+// drained, so no queued kernel still uses it).
+template <typename T> struct Scratch {
+    Part *e;
+    T *p = nullptr;
+    Scratch(Part *e_, size_t n) : e(e_) {
+        HIPCHK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+        HIPCHK(hipMemsetAsync(p, 0, (n ? n : 1) * sizeof(T), e->stream));
+    }
+    ~Scratch() {
+        if (p) {
+            (void)hipStreamSynchronize(e->stream);
+            (void)hipFree(p);
+        }
+    }
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+};
+
 // Positions of slots [s0, s0 + nb) from / to host SoA (x | y | z); the
 // device keeps them AoS, 4 doubles per slot.
 static void put_positions(Part *e, int s0, int nb, const double *r) {
@@ -954,15 +973,18 @@ static xmd_status part_get_list(Part *e, int32_t rep, int32_t id, int32_t *i1, i
         std::vector<std::pair<int, int>> out;
         auto local = [&](int s) { return s - s0; };
         if (id == XMD_LIST_NL_ALL || id == XMD_LIST_NL_NATIVE || id == XMD_LIST_NL_NON_NATIVE) {
-            int *cnt = e->alloc<int>(n);
-            k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt, nullptr, nullptr, nullptr, nullptr, 0);
-            auto c = e->get(cnt, n);
+            // scratch of this call only, freed on return
+            Scratch<int> cnt(e, n);
+            k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt.p, nullptr, nullptr, nullptr, nullptr, 0);
+            auto c = e->get(cnt.p, n);
             std::vector<int> oall(n + 1, 0);
             for (int s = 0; s < n; ++s) oall[s + 1] = oall[s] + c[s];
-            int *off = e->upload(oall);
-            int *oi1 = e->alloc<int>(oall[n] + 1), *oi2 = e->alloc<int>(oall[n] + 1);
-            uint8_t *onat = e->alloc<uint8_t>(oall[n] + 1);
-            k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt, off, oi1, oi2, onat, 1);
+            Scratch<int> off(e, n + 1), oi1_s(e, oall[n] + 1), oi2_s(e, oall[n] + 1);
+            Scratch<uint8_t> onat_s(e, oall[n] + 1);
+            e->put(off.p, oall);
+            int *oi1 = oi1_s.p, *oi2 = oi2_s.p;
+            uint8_t *onat = onat_s.p;
+            k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt.p, off.p, oi1, oi2, onat, 1);
             HIPCHK(hipGetLastError());
             auto a = e->get(oi1, oall[n]), b = e->get(oi2, oall[n]);
             auto nat = e->get(onat, oall[n]);
@@ -1543,6 +1565,22 @@ xmd_status xmd_profile(xmd_engine *e, int64_t nsteps, char *names, int64_t names
         size_t m = std::min<size_t>(all.size(), (size_t)names_cap - 1);
         std::memcpy(names, all.data(), m);
         names[m] = 0;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_observables(xmd_engine *e, double active_threshold, double *out) {
+    if (!e || !out) return XMD_ERR_ARG;
+    for (size_t g = 0; g < e->parts.size(); ++g) {
+        Part *q = e->parts[g];
+        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() {
+            Scratch<double> d(q, 8 * (size_t)q->nrep);
+            k_observables<<<q->nrep, BLK, 0, q->stream>>>(q->D, q->G, q->cur, active_threshold, d.p);
+            HIPCHK(hipGetLastError());
+            auto h = q->get(d.p, 8 * (size_t)q->nrep);
+            std::memcpy(out + 8 * (size_t)e->part_rep0[g], h.data(), h.size() * sizeof(double));
+        }));
+        if (st != XMD_OK) return st;
     }
     return XMD_OK;
 }

@@ -844,6 +844,102 @@ __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
     }
 }
 
+// ---------------------------------------------------------- observables ---
+// Sum over a workgroup in FP64, fixed order (wave shuffle tree, then waves
+// in order); every thread gets the total.
+__device__ inline double block_sum(double v, double *lds) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if (lane_id() == 0) lds[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < BLK / WAVE; ++k) t += lds[k];
+    return t;
+}
+
+// Eigenvalues of a symmetric 3x3 matrix (a00, a11, a22, a01, a02, a12),
+// ascending, by the trigonometric closed form of the characteristic cubic.
+__device__ inline void sym3_eig(const double a[6], double e[3]) {
+    const double p1 = a[3] * a[3] + a[4] * a[4] + a[5] * a[5];
+    const double q = (a[0] + a[1] + a[2]) / 3.0;
+    const double b0 = a[0] - q, b1 = a[1] - q, b2 = a[2] - q;
+    const double p2 = b0 * b0 + b1 * b1 + b2 * b2 + 2.0 * p1;
+    if (p2 <= 0.0) {
+        e[0] = e[1] = e[2] = q;
+        return;
+    }
+    const double pp = sqrt(p2 / 6.0), ip = 1.0 / pp;
+    const double c0 = b0 * ip, c1 = b1 * ip, c2 = b2 * ip, c3 = a[3] * ip, c4 = a[4] * ip, c5 = a[5] * ip;
+    double r = 0.5 * (c0 * (c1 * c2 - c5 * c5) - c3 * (c3 * c2 - c5 * c4) + c4 * (c3 * c5 - c1 * c4));
+    r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+    const double phi = acos(r) / 3.0;
+    const double hi = q + 2.0 * pp * cos(phi), lo = q + 2.0 * pp * cos(phi + 2.0 * M_PI / 3.0);
+    e[0] = lo;
+    e[1] = 3.0 * q - hi - lo;
+    e[2] = hi;
+}
+
+// Per-replica observables of the stats report (xmd/src/io/report_stats.cpp:
+// 8-41): out[8 rep + k] = t, V (last force evaluation), K = sum |v|^2 / 2
+// with v = y1 / dt (stats/total_energy.cpp:4-10), E = V + K, Rg
+// (stats/gyration.cpp:4-18, unit masses), W = 1.5 s_min^2 - 0.5 |s|^2 over
+// the singular values s of the centred positions (stats/asphericity.cpp),
+// and the native contacts active, |r1 - r2| < thr 2^(-1/6) d_nat on raw
+// positions (io/report_structure.cpp:81-85), and their number.
+__global__ __launch_bounds__(BLK) void k_observables(Dev D, Derived G, int p, double thr, double *out) {
+    __shared__ double lds[BLK / WAVE];
+    const int rep = blockIdx.x, tid = threadIdx.x, n = D.nslot;
+    const int s0 = D.rep_slot_off[rep], s1 = s0 + D.rep_n[rep];
+    const double *P = D.pos[p];
+    const double cut = thr * pow(2.0, -1.0 / 6.0);
+    double sx = 0, sy = 0, sz = 0, kk = 0, vv = 0, act = 0, tot = 0;
+    for (int s = s0 + tid; s < s1; s += BLK) {
+        const v3<double> r = ldp(P, s);
+        sx += r.x;
+        sy += r.y;
+        sz += r.z;
+        kk += norm_sq(ld3(D.y, n, s));
+        vv += D.Vbead[s];
+        for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k) {
+            const int j = D.nat_j[k];
+            if (j <= s) continue;
+            tot += 1.0;
+            act += norm(ldp(P, j) - r) < cut * D.nat_d[k] ? 1.0 : 0.0;
+        }
+    }
+    const double nb = (double)(s1 - s0);
+    const v3<double> c = {block_sum(sx, lds) / nb, block_sum(sy, lds) / nb, block_sum(sz, lds) / nb};
+    kk = block_sum(kk, lds);
+    vv = block_sum(vv, lds);
+    act = block_sum(act, lds);
+    tot = block_sum(tot, lds);
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = s0 + tid; s < s1; s += BLK) {
+        const v3<double> d = ldp(P, s) - c;
+        a[0] += d.x * d.x;
+        a[1] += d.y * d.y;
+        a[2] += d.z * d.z;
+        a[3] += d.x * d.y;
+        a[4] += d.x * d.z;
+        a[5] += d.y * d.z;
+    }
+    for (int k = 0; k < 6; ++k) a[k] = block_sum(a[k], lds);
+    if (tid == 0) {
+        double e[3];
+        sym3_eig(a, e);
+        const double K = 0.5 * kk * G.dt_inv * G.dt_inv;
+        double *o = out + 8 * rep;
+        o[0] = D.tt[p][rep];
+        o[1] = vv;
+        o[2] = K;
+        o[3] = vv + K;
+        o[4] = sqrt((a[0] + a[1] + a[2]) / nb);
+        o[5] = 1.5 * e[0] - 0.5 * (e[0] + e[1] + e[2]);
+        o[6] = act;
+        o[7] = tot;
+    }
+}
+
 // Per-bead count / fill of the full canonical Verlet list from orig_r
 // (export only, legacy_update.cpp:16-62).  mode 0 counts, 1 fills.
 __global__ __launch_bounds__(BLK) void k_nl_export(Dev D, Derived G, int *cnt, const int *off, int *oi1, int *oi2,

@@ -70,6 +70,16 @@ class Engine(HostSystem):
         self._check(lib().xmd_autotune(self.handle, C.c_int32(reps), C.byref(lpb), ms))
         return lpb.value, list(ms)
 
+    def observables(self, active_threshold=1.5):
+        """Per-replica stats-report observables computed on the GPU, one row
+        per replica: t, V, K, E, Rg, W, active native contacts, native
+        contacts (xmd/src/io/report_stats.cpp:8-41,
+        xmd/src/io/report_structure.cpp:81-85)."""
+        out = np.zeros(8 * self.nrep)
+        self._check(lib().xmd_observables(self.handle, C.c_double(active_threshold),
+                                          out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(self.nrep, 8)
+
     def rand_normals(self, state, n):
         out = np.zeros(3 * max(n, 1), dtype=np.float32)
         so = C.c_uint64()

@@ -1,6 +1,8 @@
 """Observables of a replica's state (host side, numpy).
 
   Rg: xmd/src/stats/gyration.cpp:4-18 (unit masses)
+  W:  xmd/src/stats/asphericity.cpp (singular values of the centred
+      3 x n positions, W = 1.5 s_min^2 - 0.5 |s|^2)
   K:  xmd/src/stats/total_energy.cpp:4-10
   Q:  fraction of native contacts active, |r1 - r2| < thr 2^(-1/6) d_nat
       with thr = "active threshold" (1.5),
@@ -13,6 +15,12 @@ def gyration_radius(r):
     """r: (3, n) positions."""
     com = r.mean(axis=1, keepdims=True)
     return float(np.sqrt(((r - com) ** 2).sum() / r.shape[1]))
+
+
+def asphericity(r):
+    """r: (3, n) positions."""
+    s = np.linalg.svd(r - r.mean(axis=1, keepdims=True), compute_uv=False)
+    return float(1.5 * s[-1] ** 2 - 0.5 * (s ** 2).sum())
 
 
 def kinetic_energy(v):
