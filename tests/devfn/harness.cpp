@@ -15,6 +15,7 @@ template <typename T> static TermParams<T> params_of(const double *q) {
     P.bbp_psi0 = (T)q[13]; P.bbp_alpha = (T)q[14]; P.bbp_rmin = (T)q[15]; P.bbp_depth = (T)q[16];
     P.bbm_psi0 = (T)q[17]; P.bbm_alpha = (T)q[18]; P.bbm_rmin = (T)q[19]; P.bbm_depth = (T)q[20];
     P.ss_psi0 = (T)q[21]; P.ss_alpha = (T)q[22]; P.lambda_version = (int)q[23];
+    P.set_derived();
     return P;
 }
 template <typename T> static v3<T> V(const double *g) { return {(T)g[0], (T)g[1], (T)g[2]}; }
@@ -81,8 +82,8 @@ template <typename T> static void pair_all(int kind, int n, const double *d, con
             double v = tether<double>(V<double>(d + 3 * i), (double)P.H1, (double)P.H2, par[i], o);
             f.F = {(T)o[0].x, (T)o[0].y, (T)o[0].z};
             f.V = (T)v;
-        } else if (kind == 1) f = go_pair<T>(dv, P.go_depth, (T)par[i]);
-        else if (kind == 2) f = pauli_pair<T>(dv, P.pauli_depth, P.pauli_r_excl);
+        } else if (kind == 1) f = go_pair<T>(dv, P.go_depth, P.go_d12, (T)par[i]);
+        else if (kind == 2) f = pauli_pair<T>(dv, P.pauli_depth, P.pauli_d12, P.pauli_r_excl);
         else f = es_pair<T>(dv, (T)par[i], P.es_Vf, P.es_sdi, P.es_relative);
         out[4 * i] = f.F.x;
         out[4 * i + 1] = f.F.y;

@@ -81,7 +81,10 @@ struct Dev {
     const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
     const int8_t *ptype;
     const uint8_t *bm_fix;
-    const uint64_t *jrow, *jlane, *jadv;  // jadv: rows of J^(4N) per replica
+    // rows of J^(256 2^j) (jrow), J^(256 w) for w < jwave_n (jwave) and
+    // J^(4N) per replica (jadv); lane nibble table jnib
+    const uint64_t *jrow, *jwave, *jnib, *jadv;
+    int jwave_n;
 };
 
 __device__ inline v3<double> ld3(const double *a, int n, int s) { return {a[s], a[n + s], a[2 * n + s]}; }

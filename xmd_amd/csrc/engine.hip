@@ -460,6 +460,7 @@ void derive(Part &E, const xmd_topology *t) {
         Pt.ss_psi0 = (T)p.pid_ss_psi0;
         Pt.ss_alpha = (T)p.pid_ss_alpha;
         Pt.lambda_version = p.pid_lambda_version;
+        Pt.set_derived();
     };
     fill(E.TTd);
     fill(E.TTf);
@@ -651,17 +652,34 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
     std::vector<double> charge(P.aa_charge, P.aa_charge + XMD_NUM_AA);
     std::vector<int8_t> ptype(P.aa_polarization, P.aa_polarization + XMD_NUM_AA);
     Mat J = step_matrix();
-    std::vector<uint64_t> jrow(64 * 24), jlane(64 * 64);
+    std::vector<uint64_t> jrow(64 * 24), jnib(16 * 16 * 64);
     Mat J256 = mpow(J, 256);
     for (int j = 0; j < 24; ++j) {
         rows_of(J256, jrow.data() + 64 * j);
         J256 = mul(J256, J256);
     }
+    // J^(256 w) for every wave offset of the largest replica
+    int jwave_n = 1;
+    for (int k = 0; k < E.nrep; ++k) jwave_n = std::max(jwave_n, (E.rep_n[k] + WAVE - 1) / WAVE);
+    std::vector<uint64_t> jwave(64 * (size_t)jwave_n);
+    {
+        Mat Jw = identity(), Jstep = mpow(J, 256);
+        for (int w = 0; w < jwave_n; ++w) {
+            rows_of(Jw, jwave.data() + 64 * (size_t)w);
+            Jw = mul(Jstep, Jw);
+        }
+    }
     std::vector<uint64_t> jadv(64 * (size_t)E.nrep);
     for (int k = 0; k < E.nrep; ++k) rows_of(mpow(J, 4ull * (uint64_t)E.rep_n[k]), jadv.data() + 64 * (size_t)k);
     Mat J4 = mpow(J, 4), Jl = identity();
     for (int l = 0; l < 64; ++l) {
-        for (int b = 0; b < 64; ++b) jlane[64 * b + l] = Jl.col[b];
+        for (int k = 0; k < 16; ++k)
+            for (int v = 0; v < 16; ++v) {
+                uint64_t x = 0;
+                for (int i = 0; i < 4; ++i)
+                    if ((v >> i) & 1) x ^= Jl.col[4 * k + i];
+                jnib[64 * (16 * k + v) + l] = x;
+            }
         Jl = mul(J4, Jl);
     }
 
@@ -726,7 +744,9 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
     D.charge = E.upload(charge);
     D.ptype = E.upload(ptype);
     D.jrow = E.upload(jrow);
-    D.jlane = E.upload(jlane);
+    D.jwave = E.upload(jwave);
+    D.jwave_n = jwave_n;
+    D.jnib = E.upload(jnib);
     D.jadv = E.upload(jadv);
     if (P.rng_exact) D.bm_fix = E.upload(fix_table());
     int nsl = n / WAVE;

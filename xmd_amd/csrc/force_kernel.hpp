@@ -129,7 +129,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
                 if (!D.nat_act[k]) continue;
                 int j = D.nat_j[k];
                 v3<double> d = wrap(box, ldp(Pp, j) - ri);
-                FV<T> f = go_pair<T>(cvt<T>(d), P.go_depth, (T)D.nat_d[k]);
+                FV<T> f = go_pair<T>(cvt<T>(d), P.go_depth, P.go_d12, (T)D.nat_d[k]);
                 add3(Fx, Fy, Fz, f.F);
                 if (s < j) addV(T_GO, (double)f.V);
             }
@@ -154,7 +154,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
                 const v3<T> d = cvt<T>(dd);
                 const bool owner = s < j;
                 if ((fl & F_PAULI) && (mask & (1u << T_PAULI))) {
-                    FV<T> f = pauli_pair<T>(d, P.pauli_depth, P.pauli_r_excl);
+                    FV<T> f = pauli_pair<T>(d, P.pauli_depth, P.pauli_d12, P.pauli_r_excl);
                     add3(Fx, Fy, Fz, f.F);
                     if (owner) addV(T_PAULI, (double)f.V);
                 }

@@ -28,13 +28,20 @@ __device__ inline uint64_t wave_matvec_rows(const uint64_t *rows, uint64_t x) {
     return (uint64_t)__ballot(bit);
 }
 // State of the wave's lane l at draw offset 4*(64*w + l) from S (uniform).
+// The wave offset is one mat-vec with J^(256 w) (table jwave, w < jwave_n;
+// beyond it, by the binary powers in jrow); the lane offset J^(4l) applied
+// to the uniform state is an XOR of 16 table entries, one per nibble of
+// the state: jnib[(16 k + v) 64 + l] = J^(4l) applied to v << 4k.  All 16
+// loads are independent.
 __device__ inline uint64_t wave_lane_state(const Dev &D, uint64_t S, int w) {
-    for (int j = 0; w != 0; ++j, w >>= 1)
-        if (w & 1) S = wave_matvec_rows(D.jrow + 64 * j, S);
+    if (w < D.jwave_n) S = wave_matvec_rows(D.jwave + 64 * (size_t)w, S);
+    else
+        for (int j = 0; w != 0; ++j, w >>= 1)
+            if (w & 1) S = wave_matvec_rows(D.jrow + 64 * j, S);
+    const int l = lane_id();
     uint64_t y = 0;
-    int l = lane_id();
-    for (int b = 0; b < 64; ++b)
-        if ((S >> b) & 1ull) y ^= D.jlane[64 * b + l];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) y ^= D.jnib[64 * ((k << 4) | (int)((S >> (4 * k)) & 15u)) + l];
     return y;
 }
 

@@ -151,16 +151,19 @@ XMD_HD inline v3<double> r_uv(const Box &b, v3<double> u, v3<double> v) { return
 template <typename T> struct VdV {
     T V, dV;
 };
-template <typename T> XMD_HD inline VdV<T> lj(T depth, T r_min, T r_inv) {  // lj.h:22-29
+// `d12` = T(12) * depth, the first product of dV, passed in precomputed
+// where depth is a kernel constant (the same rounding as forming it here).
+template <typename T> XMD_HD inline VdV<T> lj(T depth, T d12, T r_min, T r_inv) {  // lj.h:22-29
     T s = r_inv * r_min, s6 = ipow6(s), s12 = s6 * s6;
     T V = depth * (s12 - T(2.0) * s6);
-    T dV = T(12.0) * depth * r_inv * (s6 - s12);
+    T dV = d12 * r_inv * (s6 - s12);
     return {clampT(V, T(-1.0e3), T(1.0e3)), clampT(dV, T(-1.0e3), T(1.0e3))};
 }
-template <typename T> XMD_HD inline VdV<T> shifted_lj(T depth, T r_min, T r_inv) {  // shifted_lj.h:14-21
+template <typename T> XMD_HD inline VdV<T> lj(T depth, T r_min, T r_inv) { return lj(depth, T(12.0) * depth, r_min, r_inv); }
+template <typename T> XMD_HD inline VdV<T> shifted_lj(T depth, T d12, T r_min, T r_inv) {  // shifted_lj.h:14-21
     T s = r_inv * r_min, s6 = ipow6(s), s12 = s6 * s6;
     T V = depth * (s12 - T(2.0) * s6 + T(1.0));
-    T dV = T(12.0) * depth * r_inv * (s6 - s12);
+    T dV = d12 * r_inv * (s6 - s12);
     return {clampT(V, T(-1.0e3), T(1.0e3)), clampT(dV, T(-1.0e3), T(1.0e3))};
 }
 template <typename T> XMD_HD inline VdV<T> sink_lj(T depth, T r_min, T r_max, T r, T r_inv) {  // sink_lj.h:24-33
@@ -181,7 +184,8 @@ template <typename T> XMD_HD inline VdV<T> harmonic(T H1, T H2, T nat_r, T r) { 
 template <typename T> XMD_HD inline bool lam_supp(T psi0, T alpha, T psi) {
     return fabs(alpha * (psi - psi0)) < T(M_PI);
 }
-template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, int version, T psi) {
+// `api` = alpha / T(pi) (used by the algebraic form), precomputed.
+template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, T api, int version, T psi) {
     T s = alpha * (psi - psi0);
     if (version == 0) {
         if constexpr (std::is_same<T, float>::value) {
@@ -199,9 +203,12 @@ template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, int version
     T t = fabs(s / T(M_PI));
     T x_inv = rcpT(T(2.0f) * t * t - T(2.0f) * t - T(1));
     T val = (t * t - T(2.0f) * t + T(1.0f)) * x_inv;
-    T deriv = T(2.0f) * (t - T(1.0f)) * (t - T(2.0f)) * x_inv * x_inv * (alpha / T(M_PI));
+    T deriv = T(2.0f) * (t - T(1.0f)) * (t - T(2.0f)) * x_inv * x_inv * api;
     deriv *= (s < T(0.0f) ? T(-1.0f) : T(1.0f));
     return {val, deriv};
+}
+template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, int version, T psi) {
+    return lam_eval(psi0, alpha, alpha / T(M_PI), version, psi);
 }
 
 // ------------------------------------------------------------- params ---
@@ -216,6 +223,18 @@ template <typename T> struct TermParams {
     T bbm_psi0, bbm_alpha, bbm_rmin, bbm_depth;
     T ss_psi0, ss_alpha;
     int lambda_version;
+    // derived: T(12) * depth, alpha / T(pi) (see set_derived)
+    T go_d12, pauli_d12, bbp_d12, bbm_d12;
+    T bbp_api, bbm_api, ss_api;
+    XMD_HD void set_derived() {
+        go_d12 = T(12.0) * go_depth;
+        pauli_d12 = T(12.0) * pauli_depth;
+        bbp_d12 = T(12.0) * bbp_depth;
+        bbm_d12 = T(12.0) * bbm_depth;
+        bbp_api = bbp_alpha / T(M_PI);
+        bbm_api = bbm_alpha / T(M_PI);
+        ss_api = ss_alpha / T(M_PI);
+    }
 };
 
 // ------------------------------------------------------- pair terms ---
@@ -227,18 +246,18 @@ template <typename T> struct FV {
     T V;
 };
 // xmd/src/forces/go.cpp:31-46
-template <typename T> XMD_HD inline FV<T> go_pair(v3<T> d, T depth, T nat_dist) {
+template <typename T> XMD_HD inline FV<T> go_pair(v3<T> d, T depth, T d12, T nat_dist) {
     T rn = norm_inv(d);
     v3<T> u = d * rn;
-    VdV<T> e = lj(depth, nat_dist, rn);
+    VdV<T> e = lj(depth, d12, nat_dist, rn);
     return {u * e.dV, e.V};
 }
 // xmd/src/forces/pauli.cpp:32-48
-template <typename T> XMD_HD inline FV<T> pauli_pair(v3<T> d, T depth, T r_excl) {
+template <typename T> XMD_HD inline FV<T> pauli_pair(v3<T> d, T depth, T d12, T r_excl) {
     T rn = norm_inv(d);
     if (T(1.0f) < rn * r_excl) {
         v3<T> u = d * rn;
-        VdV<T> e = shifted_lj(depth, r_excl, rn);
+        VdV<T> e = shifted_lj(depth, d12, r_excl, rn);
         return {u * e.dV, e.V};
     }
     return {{T(0), T(0), T(0)}, T(0)};
@@ -328,27 +347,37 @@ XMD_HD inline T pid_bundle(const TermParams<T> &P, T ss_depth, T ss_rmin, T ss_r
     v3<T> r12 = cvt<T>(r12d);
     T r12_n = norm(r12), r12_rn = rcpT(r12_n);
     T A = 0, B = 0, C = 0, V = 0;
-    if (lam_supp(P.bbp_psi0, P.bbp_alpha, psi1) && lam_supp(P.bbp_psi0, P.bbp_alpha, psi2)) {
-        VdV<T> l1 = lam_eval(P.bbp_psi0, P.bbp_alpha, P.lambda_version, psi1);
-        VdV<T> l2 = lam_eval(P.bbp_psi0, P.bbp_alpha, P.lambda_version, psi2);
-        VdV<T> e = lj(P.bbp_depth, P.bbp_rmin, r12_rn);
+    // bb+ then bb-, as in the reference.  A pair is rarely in either and
+    // (with the default, disjoint supports) never in both, so the first
+    // block evaluates whichever applies with per-lane parameters and the
+    // second runs only for a pair in both; the sums keep the bb+, bb-
+    // order in every case.
+    const bool use_p = lam_supp(P.bbp_psi0, P.bbp_alpha, psi1) && lam_supp(P.bbp_psi0, P.bbp_alpha, psi2);
+    const bool use_m = lam_supp(P.bbm_psi0, P.bbm_alpha, psi1) && lam_supp(P.bbm_psi0, P.bbm_alpha, psi2);
+    if (use_p || use_m) {
+        const T psi0 = use_p ? P.bbp_psi0 : P.bbm_psi0, alpha = use_p ? P.bbp_alpha : P.bbm_alpha;
+        const T depth = use_p ? P.bbp_depth : P.bbm_depth, rmin = use_p ? P.bbp_rmin : P.bbm_rmin;
+        const T d12 = use_p ? P.bbp_d12 : P.bbm_d12, api = use_p ? P.bbp_api : P.bbm_api;
+        VdV<T> l1 = lam_eval(psi0, alpha, api, P.lambda_version, psi1);
+        VdV<T> l2 = lam_eval(psi0, alpha, api, P.lambda_version, psi2);
+        VdV<T> e = lj(depth, d12, rmin, r12_rn);
         V += l1.V * l2.V * e.V;
         A += l1.dV * l2.V * e.V;
         B += l2.dV * l1.V * e.V;
         C += l1.V * l2.V * e.dV;
     }
-    if (lam_supp(P.bbm_psi0, P.bbm_alpha, psi1) && lam_supp(P.bbm_psi0, P.bbm_alpha, psi2)) {
-        VdV<T> l1 = lam_eval(P.bbm_psi0, P.bbm_alpha, P.lambda_version, psi1);
-        VdV<T> l2 = lam_eval(P.bbm_psi0, P.bbm_alpha, P.lambda_version, psi2);
-        VdV<T> e = lj(P.bbm_depth, P.bbm_rmin, r12_rn);
+    if (use_p && use_m) {
+        VdV<T> l1 = lam_eval(P.bbm_psi0, P.bbm_alpha, P.bbm_api, P.lambda_version, psi1);
+        VdV<T> l2 = lam_eval(P.bbm_psi0, P.bbm_alpha, P.bbm_api, P.lambda_version, psi2);
+        VdV<T> e = lj(P.bbm_depth, P.bbm_d12, P.bbm_rmin, r12_rn);
         V += l1.V * l2.V * e.V;
         A += l1.dV * l2.V * e.V;
         B += l2.dV * l1.V * e.V;
         C += l1.V * l2.V * e.dV;
     }
     if (lam_supp(P.ss_psi0, P.ss_alpha, psi1) && lam_supp(P.ss_psi0, P.ss_alpha, psi2)) {
-        VdV<T> l1 = lam_eval(P.ss_psi0, P.ss_alpha, P.lambda_version, psi1);
-        VdV<T> l2 = lam_eval(P.ss_psi0, P.ss_alpha, P.lambda_version, psi2);
+        VdV<T> l1 = lam_eval(P.ss_psi0, P.ss_alpha, P.ss_api, P.lambda_version, psi1);
+        VdV<T> l2 = lam_eval(P.ss_psi0, P.ss_alpha, P.ss_api, P.lambda_version, psi2);
         VdV<T> e = sink_lj(ss_depth, ss_rmin, ss_rmax, r12_n, r12_rn);
         V += l1.V * l2.V * e.V;
         A += l1.dV * l2.V * e.V;
@@ -571,43 +600,41 @@ XMD_HD inline uint64_t rng_step(uint64_t s) {
 // normal_x2 / normal of rand_gen (rand_gen.h:42-58) with the corrected
 // float libm (bm_math.hpp); fix_table[k] holds the glibc corrections of
 // draw value k (bits 0-1 log, 2-3 sin, 4-5 cos), or nullptr to skip them.
-XMD_HD inline float bm_radius(uint32_t hi, const uint8_t *fix) {
+// `code` is the draw's byte of the fix table (0 = no correction).
+XMD_HD inline float bm_radius(uint32_t hi, uint32_t code) {
     float v = (float)hi;
     float r1 = v * (1.0f / 4294967296.0f);
-    float lg = xmd_bm::logf_cr(r1);
-    if (fix) lg = xmd_bm::apply_fix(lg, fix[xmd_bm::draw_index(v)] & 3u);
+    float lg = xmd_bm::apply_fix(xmd_bm::logf_cr(r1), code & 3u);
     float arg = -2.0f * lg;
     float r = (float)sqrt((double)arg);
     if (r != r) r = 0.0f;
     return r;
 }
-XMD_HD inline void bm_angle(uint32_t hi, const uint8_t *fix, float &s, float &c) {
+XMD_HD inline void bm_angle(uint32_t hi, uint32_t code, float &s, float &c) {
     float v = (float)hi;
     float r2 = v * (1.0f / 4294967296.0f);
     float t = (float)(2.0 * M_PI) * r2;
     double sd, cd;
     xmd_bm::sincos_red((double)t, sd, cd);
-    s = (float)sd;
-    c = (float)cd;
-    if (fix) {
-        uint32_t code = fix[xmd_bm::draw_index(v)];
-        s = xmd_bm::apply_fix(s, (code >> 2) & 3u);
-        c = xmd_bm::apply_fix(c, (code >> 4) & 3u);
-    }
+    s = xmd_bm::apply_fix((float)sd, (code >> 2) & 3u);
+    c = xmd_bm::apply_fix((float)cd, (code >> 4) & 3u);
 }
-// The three noise components of one bead from its first state.
+// The three noise components of one bead from its first state.  The four
+// fix-table bytes are fetched before any arithmetic so their loads overlap.
 XMD_HD inline void bead_noise(uint64_t s, const uint8_t *fix, float out[3]) {
-    uint32_t u[4];
+    uint32_t u[4], code[4] = {0u, 0u, 0u, 0u};
     for (int k = 0; k < 4; ++k) {
         u[k] = (uint32_t)(rng_out(s) >> 32);
         s = rng_step(s);
     }
-    float r = bm_radius(u[0], fix), sn, cs;
-    bm_angle(u[1], fix, sn, cs);
+    if (fix)
+        for (int k = 0; k < 4; ++k) code[k] = fix[xmd_bm::draw_index((float)u[k])];
+    float r = bm_radius(u[0], code[0]), sn, cs;
+    bm_angle(u[1], code[1], sn, cs);
     out[0] = r * cs;
     out[1] = r * sn;
-    float r3 = bm_radius(u[2], fix), sn3, cs3;
-    bm_angle(u[3], fix, sn3, cs3);
+    float r3 = bm_radius(u[2], code[2]), sn3, cs3;
+    bm_angle(u[3], code[3], sn3, cs3);
     out[2] = r3 * cs3;
 }
 
