@@ -85,6 +85,22 @@ def rand_normals(state, n):
     return out.reshape(n, 3), so.value
 
 
+def morph_into_saw(state, chain_lens, orig=None, bond=None, density=1.0, infer_box=False):
+    """Positions (n, 3), box and generator state after morphing chains of the
+    given lengths (model.cpp:92-174)."""
+    n = int(sum(chain_lens))
+    lens = np.asarray(chain_lens, dtype=np.int32)
+    o = np.zeros((n, 3)) if orig is None else np.ascontiguousarray(orig, dtype=np.float64)
+    pos = np.zeros((n, 3))
+    box = np.zeros(3)
+    s = C.c_uint64(state)
+    lib().oracle_morph_into_saw(C.byref(s), C.c_int(len(lens)), lens.ctypes.data_as(C.POINTER(C.c_int)),
+                                o.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(-1.0 if bond is None else bond),
+                                C.c_double(density), C.c_int(int(infer_box)),
+                                pos.ctypes.data_as(C.POINTER(C.c_double)), box.ctypes.data_as(C.POINTER(C.c_double)))
+    return pos, box, s.value
+
+
 def prim(kind, *args):
     a = np.array(args, dtype=np.float64)
     out = np.zeros(2)

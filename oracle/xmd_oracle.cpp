@@ -55,6 +55,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <set>
 #include <string>
 #include <utility>
@@ -219,6 +220,27 @@ struct RandGen {
         if (std::isnan(r)) r = 0.0f;
         float t = (float)(2.0 * M_PI) * r2;
         return {r * std::cos(t), r * std::sin(t)};
+    }
+    // double forms (uniform<double>, uniform(a, b), normal_x2, normal, sphere)
+    double uniformd() {
+        double inv = 1.0 / (double)(1ull << 32);
+        return (double)(next() >> 32ull) * inv;
+    }
+    double uniformd(double a, double b) { return (b - a) * uniformd() + a; }
+    void sphere(double v[3]) {
+        double r1 = uniformd(), r2 = uniformd();
+        double r = std::sqrt(-2.0 * std::log(r1));
+        if (std::isnan(r)) r = 0.0;
+        double t = (2.0 * M_PI) * r2;
+        v[0] = r * std::cos(t);
+        v[1] = r * std::sin(t);
+        r1 = uniformd(), r2 = uniformd();
+        r = std::sqrt(-2.0 * std::log(r1));
+        if (std::isnan(r)) r = 0.0;
+        t = (2.0 * M_PI) * r2;
+        v[2] = r * std::cos(t);
+        double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; ++k) v[k] /= n;
     }
 };
 
@@ -1477,6 +1499,65 @@ void oracle_rand_normals(uint64_t state, int64_t n, float *out, uint64_t *state_
         out[3 * k + 2] = z;
     }
     if (state_out) *state_out = g.state;
+}
+
+// xmd/src/model/model.cpp:92-174: chains laid out as random walks; the
+// rotations are applied as Rodrigues matrices R(axis, angle).
+static void rodrigues(const double ax[3], double ang, double R[3][3]) {
+    double c = std::cos(ang), s = std::sin(ang), t = 1.0 - c;
+    double x = ax[0], y = ax[1], z = ax[2];
+    R[0][0] = c + x * x * t, R[0][1] = x * y * t - z * s, R[0][2] = x * z * t + y * s;
+    R[1][0] = y * x * t + z * s, R[1][1] = c + y * y * t, R[1][2] = y * z * t - x * s;
+    R[2][0] = z * x * t - y * s, R[2][1] = z * y * t + x * s, R[2][2] = c + z * z * t;
+}
+void oracle_morph_into_saw(uint64_t *state, int n_chains, const int *chain_len, const double *orig, double bond,
+                           double density, int infer_box, double *pos, double *box) {
+    RandGen g{*state};
+    int n = 0;
+    for (int c = 0; c < n_chains; ++c) n += chain_len[c];
+    const double a = std::cbrt((double)n / density);
+    int base = 0;
+    for (int c = 0; c < n_chains; ++c) {
+        double p[3], d[3];
+        for (int k = 0; k < 3; ++k) p[k] = g.uniformd(-a / 2.0, a / 2.0);
+        g.sphere(d);
+        for (int i = 0; i < chain_len[c]; ++i) {
+            double nx[3] = {p[0], p[1], p[2]};
+            if (i + 1 < chain_len[c]) {
+                double bl = bond;
+                if (!(bond > 0.0)) {
+                    const double *q1 = orig + 3 * (base + i), *q2 = orig + 3 * (base + i + 1);
+                    bl = std::sqrt((q2[0] - q1[0]) * (q2[0] - q1[0]) + (q2[1] - q1[1]) * (q2[1] - q1[1]) +
+                                   (q2[2] - q1[2]) * (q2[2] - q1[2]));
+                }
+                for (int k = 0; k < 3; ++k) nx[k] += d[k] * bl;
+                double spread = g.uniformd(-M_PI / 12.0, M_PI / 12.0);
+                // perpendicular: the larger of d x e_x, d x e_y, normalised
+                double px[3] = {0.0, d[2], -d[1]}, py[3] = {-d[2], 0.0, d[0]};
+                double nxn = std::sqrt(px[1] * px[1] + px[2] * px[2]), nyn = std::sqrt(py[0] * py[0] + py[2] * py[2]);
+                double perp[3];
+                for (int k = 0; k < 3; ++k) perp[k] = nxn > nyn ? px[k] / nxn : py[k] / nyn;
+                double around = g.uniformd(-M_PI, M_PI);
+                double Rs[3][3], Ra[3][3], t[3], u[3];
+                rodrigues(perp, spread, Rs);
+                rodrigues(d, around, Ra);
+                for (int r = 0; r < 3; ++r) t[r] = Rs[r][0] * d[0] + Rs[r][1] * d[1] + Rs[r][2] * d[2];
+                for (int r = 0; r < 3; ++r) u[r] = Ra[r][0] * t[0] + Ra[r][1] * t[1] + Ra[r][2] * t[2];
+                double un = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                for (int k = 0; k < 3; ++k) d[k] = u[k] / un;
+            }
+            for (int k = 0; k < 3; ++k) pos[3 * (base + i) + k] = p[k];
+            for (int k = 0; k < 3; ++k) p[k] = nx[k];
+        }
+        base += chain_len[c];
+    }
+    if (infer_box) {
+        // every extent from x; maxima start at the smallest positive double
+        double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::min();
+        for (int i = 0; i < n; ++i) lo = std::min(lo, pos[3 * i]), hi = std::max(hi, pos[3 * i]);
+        box[0] = box[1] = box[2] = hi - lo;
+    }
+    *state = g.state;
 }
 
 void oracle_prim(int kind, const double *a, double *out) {

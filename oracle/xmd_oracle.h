@@ -58,6 +58,11 @@ void oracle_rand_normals(uint64_t state, int64_t n, float *out,
                          uint64_t *state_out);
 /* Pure primitives for closed-form tests: out = {V, dV/dr}. */
 void oracle_prim(int kind, const double *args, double *out);
+// chains of the given lengths laid out as random walks (model.cpp:92-174);
+// bond <= 0 takes each bond length from orig (3 per residue); box is
+// written when infer_box is set; *state advances past the draws
+void oracle_morph_into_saw(uint64_t *state, int n_chains, const int *chain_len, const double *orig, double bond,
+                           double density, int infer_box, double *pos, double *box);
 
 #ifdef __cplusplus
 }

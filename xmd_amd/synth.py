@@ -359,6 +359,24 @@ def replicate(system, temperatures, seeds):
         nat_chir=tile(system.nat_chir), nat_chir_factor=tile(system.nat_chir_factor))
 
 
+def concat(systems):
+    """Systems (of any replica counts) as the replicas of one batch, in order."""
+    off = np.cumsum([0] + [s.n_beads for s in systems])
+    cat = lambda name: np.concatenate([getattr(s, name) for s in systems])
+    shift = lambda name: np.concatenate([getattr(s, name) + o for s, o in zip(systems, off)])
+    return System(
+        replica_offset=np.concatenate([s.replica_offset[:-1] + o for s, o in zip(systems, off)] + [[off[-1]]])
+        .astype(np.int32),
+        chain_offset=np.concatenate([s.chain_offset[:-1] + o for s, o in zip(systems, off)] + [[off[-1]]])
+        .astype(np.int32),
+        atype=cat("atype"), r0=np.concatenate([s.r0 for s in systems], axis=1),
+        box=np.concatenate([s.box.reshape(-1, 3) for s in systems]), temperature=cat("temperature"),
+        rng_state=cat("rng_state").astype(np.uint64),
+        nat_i1=shift("nat_i1").astype(np.int32), nat_i2=shift("nat_i2").astype(np.int32), nat_dist=cat("nat_dist"),
+        tether_len=cat("tether_len"), nat_theta=cat("nat_theta"), nat_phi=cat("nat_phi"),
+        nat_chir=cat("nat_chir"), nat_chir_factor=cat("nat_chir_factor"))
+
+
 def perturbed_positions(system, seed, sigma_A=0.3):
     """Native positions plus isotropic Gaussian displacement (C1's seeded
     conformations)."""
