@@ -61,7 +61,8 @@ typedef enum xmd_term {
     XMD_TERM_REL_ES = 8,    /* "relative ES"          src/forces/es/relative.cpp:13-32       */
     XMD_TERM_QA = 9,        /* "quasi-adiabatic potential" src/forces/qa/process_contacts.cpp:13-57 */
     XMD_TERM_PID = 10,      /* "pseudo-improper dihedral potential" src/forces/pid/eval_forces.cpp:14-141 */
-    XMD_NUM_TERMS = 11
+    XMD_TERM_NAT_SS = 11,   /* "native ssbonds"       src/forces/native_ss.cpp:27-40         */
+    XMD_NUM_TERMS = 12
 } xmd_term;
 
 /* Lists produced by a Verlet rebuild (xmd/include/xmd/nl/data.h:8-14 and the
@@ -74,7 +75,9 @@ typedef enum xmd_list_id {
     XMD_LIST_PAULI = 4,         /* update_pauli_pairs output                */
     XMD_LIST_ES = 5,            /* update_es_base output (q1*q2 != 0)       */
     XMD_LIST_PID = 6,           /* update_pid_bundles output                */
-    XMD_LIST_QA_FREE = 7        /* QA free pairs (non-vacant, set order)    */
+    XMD_LIST_QA_FREE = 7,       /* QA free pairs (non-vacant, set order)    */
+    XMD_LIST_NL_NAT_SS = 8,     /* nl_data::native_ssbonds                  */
+    XMD_LIST_NAT_SS = 9         /* update_nat_ssbonds output                */
 } xmd_list_id;
 
 typedef enum xmd_precision {
@@ -153,6 +156,9 @@ typedef struct xmd_params {
     int32_t force_lanes;        /* lanes per bead of the force kernel: 0 = 8,
                                    or 4, 8, 16 (xmd_autotune picks one by
                                    timing); no counterpart in freeform_params */
+    /* "native ssbonds" (params.cpp:180-184) */
+    int32_t nat_ss_enabled;
+    double nat_ss_H1, nat_ss_equil_dist;
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and
@@ -179,6 +185,11 @@ typedef struct xmd_topology {
     const double *nat_phi;        /* [n_beads] dihedral i..i+3                 */
     const double *nat_chir;       /* [n_beads] chirality i..i+3                */
     const double *nat_chir_factor;/* [n_beads] 1/|r23_nat|^3                   */
+    /* native disulfide bonds (model contacts of type NAT_SS), global,
+       i1 < i2 within one replica, sorted per replica, disjoint from nat_* */
+    int32_t n_ssbonds;
+    const int32_t *ss_i1;
+    const int32_t *ss_i2;
 } xmd_topology;
 
 /* QA contact record (xmd/include/xmd/forces/qa/contact.h:15-16). */

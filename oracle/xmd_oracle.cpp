@@ -48,6 +48,13 @@
 //       (main.cpp:393 passes them swapped).
 //   D19 algebraic lambda: value as written, derivative = d(value)/d(psi)
 //       (lambda.h:47 omits alpha and has the wrong polynomial).
+//   D23 current disulfide bonds: pairs within 2 * equilibrium distance
+//       (harmonic::cutoff, the value main.cpp:592 adds to max_cutoff) plus
+//       the list pad; update_nat_ssbonds::cutoff is never set
+//       (native_ss.h:44, main.cpp:587-591).
+//   D24 disulfide pairs of the list (native_ssbonds) belong to neither
+//       native_contacts nor non_native, so no pair term but the bond sees
+//       them (Debye-Hueckel reads native_contacts + non_native, D2).
 #include "xmd_oracle.h"
 
 #include <algorithm>
@@ -263,7 +270,8 @@ inline Sync add(Sync a, Sync b) {
 template <typename R> struct NlPair {
     int i1, i2;
     R orig_dist;
-    bool native;
+    bool native;  // native_contacts or native_ssbonds (not non_native)
+    bool ss;      // native_ssbonds
 };
 template <typename R> struct NatCont {
     int i1, i2;
@@ -318,7 +326,8 @@ template <typename R> struct Params {
     double dt;
     R gamma_factor;
     R pad_factor, max_cutoff;
-    bool tether_on, chir_on, ang_on, cnd_on, snd_on, pauli_on, go_on, ces_on, res_on, qa_on, pid_on;
+    bool tether_on, chir_on, ang_on, cnd_on, snd_on, pauli_on, go_on, ces_on, res_on, qa_on, pid_on, ss_on;
+    R ss_H1, ss_equil;
     R H1, H2, def_len, e_chi, ang_k, CDA, CDB, CDH;
     R pauli_r_excl, pauli_depth, go_depth;
     R ces_Vf, ces_sdi, res_Vf, res_sdi, es_cutoff;
@@ -364,6 +373,7 @@ template <typename R> struct Replica {
     Box<R> orig_box;
     std::vector<NatCont<R>> all_nat, go_cur;
     std::vector<std::pair<int, int>> all_nat_nl;  // sorted (i1,i2)
+    std::vector<std::pair<int, int>> all_ss, ss_cur;  // sorted (i1,i2); current
     std::vector<std::pair<int, int>> pauli;
     std::vector<EsPair<R>> es;
     std::vector<PidBundle> pid;
@@ -505,6 +515,18 @@ template <typename R> void eval_go(const Params<R> &P, Replica<R> &s, std::vecto
         *V += V_;
         F[c.i1] += r12_u * dV_dr;
         F[c.i2] -= r12_u * dV_dr;
+    }
+}
+template <typename R> void eval_nat_ss(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, size_t lo = 0, size_t hi = SIZE_MAX) {
+    for (size_t k = lo; k < std::min(hi, s.ss_cur.size()); ++k) {
+        int i1 = s.ss_cur[k].first, i2 = s.ss_cur[k].second;  // xmd/src/forces/native_ss.cpp:27-40
+        V3<R> r12 = s.box.r_uv(s.r[i1], s.r[i2]);
+        R r12_n = norm(r12);
+        auto [V_, dV_dr] = Harmonic<R>{P.ss_H1, R(0.0f), P.ss_equil}(r12_n);
+        V3<R> r12_u = r12 / r12_n;
+        *V += V_;
+        F[i1] += dV_dr * r12_u;
+        F[i2] -= dV_dr * r12_u;
     }
 }
 template <typename R> void eval_es(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, bool relative, size_t lo = 0,
@@ -716,7 +738,7 @@ template <typename R> void legacy_update(const Params<R> &P, Replica<R> &s) {  /
     R pad = P.pad_factor * P.max_cutoff;
     R req_r = P.max_cutoff + pad;
     s.nl.clear();
-    size_t nat_idx = 0;
+    size_t nat_idx = 0, ss_idx = 0;
     for (int i1 = 0; i1 < s.n; ++i1) {
         V3<R> r1 = s.r[i1];
         int chain1 = s.chain[i1], seq1 = s.seq[i1];
@@ -729,7 +751,9 @@ template <typename R> void legacy_update(const Params<R> &P, Replica<R> &s) {  /
                 std::pair<int, int> p{i1, i2};
                 while (nat_idx < s.all_nat_nl.size() && s.all_nat_nl[nat_idx] < p) ++nat_idx;
                 bool native = nat_idx < s.all_nat_nl.size() && s.all_nat_nl[nat_idx] == p;
-                s.nl.push_back({i1, i2, orig_dist, native});
+                while (ss_idx < s.all_ss.size() && s.all_ss[ss_idx] < p) ++ss_idx;
+                bool ss = ss_idx < s.all_ss.size() && s.all_ss[ss_idx] == p;
+                s.nl.push_back({i1, i2, orig_dist, native || ss, ss});
             }
         }
     }
@@ -748,6 +772,12 @@ template <typename R> void update_lists(const Params<R> &P, Replica<R> &s) {
             if (norm(s.box.r_uv(s.r[c.i1], s.r[c.i2])) < cutoff + pad) s.go_cur.push_back(c);
         }
     }
+    if (P.ss_on) {  // xmd/src/forces/native_ss.cpp:8-19 (+D23: cutoff = harmonic::cutoff(equil))
+        s.ss_cur.clear();
+        R cutoff = R(2.0) * P.ss_equil;
+        for (auto const &c : s.all_ss)
+            if (norm(s.box.r_uv(s.r[c.first], s.r[c.second])) < cutoff + pad) s.ss_cur.push_back(c);
+    }
     if (P.pauli_on) {  // xmd/src/forces/pauli.cpp:9-24 (+D2,D10)
         s.pauli.clear();
         R min_norm_inv = R(1.0) / (P.pauli_r_excl + pad);
@@ -760,6 +790,7 @@ template <typename R> void update_lists(const Params<R> &P, Replica<R> &s) {
         s.es.clear();
         R min_norm_inv = R(1.0) / (P.es_cutoff + pad);
         for (auto const &p : s.nl) {
+            if (p.ss) continue;
             if (norm_inv(s.box.r_uv(s.r[p.i1], s.r[p.i2])) > min_norm_inv) {
                 R q1_x_q2 = P.q[s.atype[p.i1]] * P.q[s.atype[p.i2]];
                 if (q1_x_q2 != 0.0) s.es.push_back({p.i1, p.i2, q1_x_q2});
@@ -876,6 +907,7 @@ template <typename R> struct System : SysBase {
         if (P.snd_on && on(XMD_TERM_SND)) eval_snd(P, s, F, V(XMD_TERM_SND));
         if (P.pauli_on && on(XMD_TERM_PAULI)) eval_pauli(P, s, F, V(XMD_TERM_PAULI));
         if (P.go_on && on(XMD_TERM_GO)) eval_go(P, s, F, V(XMD_TERM_GO));
+        if (P.ss_on && on(XMD_TERM_NAT_SS)) eval_nat_ss(P, s, F, V(XMD_TERM_NAT_SS));
         if (P.ces_on && on(XMD_TERM_CONST_ES)) eval_es(P, s, F, V(XMD_TERM_CONST_ES), false);
         if (P.res_on && on(XMD_TERM_REL_ES)) eval_es(P, s, F, V(XMD_TERM_REL_ES), true);
         if (P.qa_on && on(XMD_TERM_QA)) {
@@ -973,6 +1005,7 @@ template <typename R> struct System : SysBase {
         if (P.cnd_on || P.snd_on) { chunk(s.dihs.size(), lo, hi); range_eval(s, &Replica<R>::dihs, lo, hi, F, V, 2); }
         if (P.pauli_on) { chunk(s.pauli.size(), lo, hi); range_eval(s, &Replica<R>::pauli, lo, hi, F, V, 3); }
         if (P.go_on) { chunk(s.go_cur.size(), lo, hi); range_eval(s, &Replica<R>::go_cur, lo, hi, F, V, 4); }
+        if (P.ss_on) { chunk(s.ss_cur.size(), lo, hi); range_eval(s, &Replica<R>::ss_cur, lo, hi, F, V, 9); }
         if (P.ces_on || P.res_on) { chunk(s.es.size(), lo, hi); range_eval(s, &Replica<R>::es, lo, hi, F, V, 5); }
         if (P.qa_on) {
 #pragma omp barrier
@@ -1015,6 +1048,7 @@ template <typename R> struct System : SysBase {
         case 6: eval_pid(P, s, F, V, lo, hi); break;
         case 7: qa_process_contacts(P, s, F, V, false, lo, hi); break;
         case 8: eval_chiral(P, s, F, V, lo, hi); break;
+        case 9: eval_nat_ss(P, s, F, V, lo, hi); break;
         }
     }
 #endif
@@ -1037,6 +1071,9 @@ template <typename R> int setup_params(const xmd_params *p, Params<R> &P) {
     P.res_on = p->rel_es_enabled;
     P.qa_on = p->qa_enabled;
     P.pid_on = p->pid_enabled;
+    P.ss_on = p->nat_ss_enabled;
+    P.ss_H1 = (R)p->nat_ss_H1;
+    P.ss_equil = (R)p->nat_ss_equil_dist;
     P.H1 = (R)p->tether_H1;
     P.H2 = (R)p->tether_H2;
     P.def_len = (R)p->tether_def_length;
@@ -1117,6 +1154,7 @@ template <typename R> int build_system(const xmd_params *p, const xmd_topology *
     if (P.ces_on || P.res_on) mc = std::max(mc, P.es_cutoff);
     if (P.qa_on) mc = std::max(mc, P.max_formation_min_dist);
     if (P.pid_on) mc = std::max(mc, P.pid_cutoff);
+    if (P.ss_on) mc = std::max(mc, R(2.0) * P.ss_equil);  // main.cpp:592-593
     if (p->max_cutoff_override > 0) mc = (R)p->max_cutoff_override;
     P.max_cutoff = mc;
 
@@ -1176,6 +1214,17 @@ template <typename R> int build_system(const xmd_params *p, const xmd_topology *
             s.all_nat_nl.push_back({a - off, b - off});
         }
         if (!std::is_sorted(s.all_nat_nl.begin(), s.all_nat_nl.end())) return fail("native contacts not sorted");
+        // disulfide bonds of this replica (main.cpp:560-577)
+        for (int c = 0; c < t->n_ssbonds; ++c) {
+            int a = t->ss_i1[c], b = t->ss_i2[c];
+            if (a < off || a >= off + n) continue;
+            if (b < off || b >= off + n || !(a < b)) return fail("ssbond crosses replicas or i1 >= i2");
+            s.all_ss.push_back({a - off, b - off});
+        }
+        if (!std::is_sorted(s.all_ss.begin(), s.all_ss.end())) return fail("ssbonds not sorted");
+        for (auto const &q : s.all_ss)
+            if (std::binary_search(s.all_nat_nl.begin(), s.all_nat_nl.end(), q))
+                return fail("pair is both a native contact and an ssbond");
         // bonded lists: consecutive beads of each chain (model tethers,
         // angles, dihedrals), main.cpp:377-479
         for (int i = 0; i < n; ++i) {
@@ -1317,8 +1366,13 @@ int oracle_get_list(oracle_sys *h, int replica, int id, int32_t *i1, int32_t *i2
             break;
         case XMD_LIST_NL_NATIVE:
             for (auto &p : s.nl)
-                if (p.native) out.push_back({p.i1, p.i2});
+                if (p.native && !p.ss) out.push_back({p.i1, p.i2});
             break;
+        case XMD_LIST_NL_NAT_SS:
+            for (auto &p : s.nl)
+                if (p.ss) out.push_back({p.i1, p.i2});
+            break;
+        case XMD_LIST_NAT_SS: out = s.ss_cur; break;
         case XMD_LIST_NL_NON_NATIVE:
             for (auto &p : s.nl)
                 if (!p.native) out.push_back({p.i1, p.i2});

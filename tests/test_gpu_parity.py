@@ -352,3 +352,45 @@ def test_observables_match_host(prec):
         assert abs(obs[k, 5] - asphericity(r)) <= 1e-9 * s2
         assert obs[k, 7] == m.sum() and abs(obs[k, 6] - act) <= 1, (obs[k, 6], act)
     eng.close()
+
+
+def ss_system():
+    """A 40-bead chain with three disulfide bonds: two within reach, one
+    beyond 2 r0 + pad."""
+    import dataclasses
+    base = synth.config_system(1, chain_len=40)
+    r = base.r0.copy()
+    r[:, 2] = r[:, 20] + np.array([1.1, 0.3, -0.2])
+    r[:, 30] = r[:, 5] + np.array([0.0, 1.4, 0.5])
+    r[:, 39] = r[:, 10] + np.array([6.0, 0.0, 0.0])
+    system = dataclasses.replace(base, r0=r, ss_i1=np.array([2, 5, 10], dtype=np.int32),
+                                 ss_i2=np.array([20, 30, 39], dtype=np.int32))
+    params = synth.config_params(1)
+    params.nat_ss_enabled = 1
+    return params, system
+
+
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_MIXED])
+def test_nat_ss_bonds(prec):
+    params, system = ss_system()
+    params.precision = prec
+    eng = Engine(params, system)
+    params.precision = _abi.PREC_FP64
+    ora = Oracle(params, system)
+    ora.rebuild_lists()
+    lists_equal(eng, ora, 0)
+    assert set(zip(*eng.get_list(0, "nat_ss"))) == {(2, 20), (5, 30)}
+    if prec == _abi.PREC_FP64:
+        compare_terms(eng, ora)
+        eng.step(20)
+        ora.step(20)
+        y, _, rng, t = eng.get_state(0)
+        yo, _, rngo, to = ora.get_state(0)
+        assert rng == rngo and t == to
+        assert np.abs(y - yo).max() < 1e-9
+    else:
+        k = _abi.TERM["nat_ss"]
+        F, V = eng.eval_forces(1 << k)
+        Fo, Vo = ora.eval_forces(1 << k)
+        assert rel_rms(F, Fo) < 1e-12 and abs(V[0, k] - Vo[0, k]) < 1e-12 * abs(Vo[0, k])
+    eng.close()

@@ -118,6 +118,7 @@ def test_pdb_records_contacts_and_end():
     # the Go contacts of the topology leave out the SS bond
     s = M.to_system(m, 0.35, 1)
     assert list(zip(s.nat_i1, s.nat_i2)) == [(2, 4)]
+    assert list(zip(s.ss_i1, s.ss_i2)) == [(1, 3)]
 
 
 def test_rand_gen_matches_oracle():

@@ -150,7 +150,7 @@ def _single_term(cid, term, **kw):
 def synth_key(term):
     return {"tether": "tether", "chiral": "chir", "nat_ang": "nat_ang", "cnd": "cnd", "snd": "snd",
             "pauli": "pauli", "go": "go", "const_es": "const_es", "rel_es": "rel_es", "qa": "qa",
-            "pid": "pid"}[term]
+            "pid": "pid", "nat_ss": "nat_ss"}[term]
 
 
 @pytest.fixture(scope="module")
@@ -369,3 +369,50 @@ def test_stationary_velocity_variance():
         _, vel, _, _ = ora.get_state(0)
         acc.append((vel ** 2).mean())
     assert np.mean(acc) == pytest.approx(gear_stationary_velocity_variance(2.0, 0.4, p.dt), rel=0.03)
+
+
+# ------------------------------------------------------ disulfide bonds ---
+def test_nat_ss_bonds(small_open):
+    """Native disulfide bonds (native_ss.cpp): V = H1 (r - r0)^2 / 2 summed
+    over the bonds within 2 r0 + pad (D23), F = -grad V, and the bonded
+    pairs sit in the list's native_ssbonds only, outside every pair term
+    (D24)."""
+    import dataclasses
+    base = small_open
+    n = base.n_beads
+    r = base.r0.copy()
+    # three bonds: two within reach, one pulled far apart
+    pairs = [(2, 20), (5, 30), (10, 39)]
+    ctr = r[:, 20].copy()
+    r[:, 2] = ctr + np.array([1.1, 0.3, -0.2])  # ~5.8 A from bead 20
+    r[:, 30] = r[:, 5] + np.array([0.0, 1.4, 0.5])
+    r[:, 39] = r[:, 10] + np.array([6.0, 0.0, 0.0])  # 30 A: beyond 2 r0 + pad
+    system = dataclasses.replace(base, ss_i1=np.array([a for a, _ in pairs], dtype=np.int32),
+                                 ss_i2=np.array([b for _, b in pairs], dtype=np.int32))
+    p = _single_term(1, "nat_ss")
+    p.pauli_enabled = 1  # Pauli must not see the bonded pairs
+    ora = Oracle(p, system)
+    ora.set_positions(0, r)
+    ora.rebuild_lists()
+    cur = set(zip(*ora.get_list(0, "nat_ss")))
+    assert cur == {(2, 20), (5, 30)}
+    in_nl = set(zip(*ora.get_list(0, "nl_nat_ss")))
+    assert in_nl <= set(pairs) and (2, 20) in in_nl
+    for name in ("nl_native", "nl_non_native", "pauli"):
+        assert not (set(zip(*ora.get_list(0, name))) & set(pairs)), name
+    F, V = ora.eval_forces(1 << _abi.TERM["nat_ss"])
+    H1, r0 = p.nat_ss_H1, p.nat_ss_equil_dist
+    want = sum(0.5 * H1 * (np.linalg.norm(r[:, j] - r[:, i]) - r0) ** 2 for i, j in [(2, 20), (5, 30)])
+    assert V[0, _abi.TERM["nat_ss"]] == pytest.approx(want, rel=1e-13)
+    h = 1e-6
+    for i in (2, 20, 5, 30):
+        for a in range(3):
+            rp = r.copy(); rp[a, i] += h
+            ora.set_positions(0, rp); ora.rebuild_lists()
+            _, Vp = ora.eval_forces(1 << _abi.TERM["nat_ss"])
+            rm = r.copy(); rm[a, i] -= h
+            ora.set_positions(0, rm); ora.rebuild_lists()
+            _, Vm = ora.eval_forces(1 << _abi.TERM["nat_ss"])
+            fd = -(Vp[0, _abi.TERM["nat_ss"]] - Vm[0, _abi.TERM["nat_ss"]]) / (2 * h)
+            assert F[a, i] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+    assert np.abs(F[:, 39]).max() == 0 and np.abs(F[:, 10]).max() == 0

@@ -16,13 +16,13 @@ OK, ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_NONFINITE, ERR_STATE, ERR_NO_DEVICE = ra
 
 # xmd_term
 TERMS = ["tether", "chiral", "nat_ang", "cnd", "snd", "pauli", "go",
-         "const_es", "rel_es", "qa", "pid"]
+         "const_es", "rel_es", "qa", "pid", "nat_ss"]
 TERM = {name: k for k, name in enumerate(TERMS)}
 NUM_TERMS = len(TERMS)
 ALL_TERMS = (1 << NUM_TERMS) - 1
 
 # xmd_list_id
-LISTS = ["nl_all", "nl_native", "nl_non_native", "go", "pauli", "es", "pid", "qa_free"]
+LISTS = ["nl_all", "nl_native", "nl_non_native", "go", "pauli", "es", "pid", "qa_free", "nl_nat_ss", "nat_ss"]
 LIST = {name: k for k, name in enumerate(LISTS)}
 
 PREC_FP64, PREC_MIXED = 0, 1
@@ -61,6 +61,7 @@ class XmdParams(C.Structure):
         ("aa_limits", (_i8 * 4) * NUM_AA),
         ("aa_polarization", _i8 * NUM_AA),
         ("force_lanes", _i32),
+        ("nat_ss_enabled", _i32), ("nat_ss_H1", _d), ("nat_ss_equil_dist", _d),
     ]
 
 
@@ -73,6 +74,7 @@ class XmdTopology(C.Structure):
         ("nat_i1", C.POINTER(_i32)), ("nat_i2", C.POINTER(_i32)), ("nat_dist", C.POINTER(_d)),
         ("tether_len", C.POINTER(_d)), ("nat_theta", C.POINTER(_d)), ("nat_phi", C.POINTER(_d)),
         ("nat_chir", C.POINTER(_d)), ("nat_chir_factor", C.POINTER(_d)),
+        ("n_ssbonds", _i32), ("ss_i1", C.POINTER(_i32)), ("ss_i2", C.POINTER(_i32)),
     ]
 
 

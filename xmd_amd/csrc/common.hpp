@@ -12,12 +12,14 @@ using namespace xmd_dev;
 constexpr int WAVE = 64;
 constexpr int BLK = 256;
 constexpr int MAXQ = 16;
-constexpr int NUM_TERMS = 11;
+constexpr int NUM_TERMS = 12;
 
-enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16 };
+// F_NATSS marks a disulfide pair of the list (native_ssbonds): F_NATIVE is
+// set too, so no non-native term takes it.
+enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16, F_NATSS = 32 };
 enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
 enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
-enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID };
+enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID, T_NAT_SS };
 
 // Derived parameters in double (list and QA decisions, integrator).
 struct Derived {
@@ -27,6 +29,10 @@ struct Derived {
     double pauli_min_inv, es_min_inv, pid_min_inv, qa_min_inv;
     int pauli_on, es_on, pid_on, qa_on, go_on;
     int lang_on;
+    // disulfide bonds: harmonic H1 about the equilibrium distance, current
+    // while within ss_cut + pad (FP64 in both forms)
+    int ss_on;
+    double ss_H1, ss_equil, ss_cut;
     QaParams qa;
 };
 
@@ -56,6 +62,9 @@ struct Dev {
     const int *nat_start, *nat_j;
     const double *nat_d;
     uint8_t *nat_act;
+    // disulfide bonds (rows of slot s: ss_start[s]..ss_start[s+1])
+    const int *ss_start, *ss_j;
+    uint8_t *ss_act;
     // neighbour rows
     int nl_w;  // row capacity per slot (rows are fixed-width sliced ELL)
     int *slice_start, *nl_len, *nl_j;

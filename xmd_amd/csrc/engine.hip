@@ -376,7 +376,9 @@ struct Part {
 namespace {
 
 // Derived constants, main.cpp:345-907 expressions in double.
-void derive(Part &E, const xmd_topology *t) {
+// go_cut: 2 * the longest native contact length over the whole topology
+// (every stream group gets the same list radius).
+void derive(Part &E, double go_cut) {
     const xmd_params &p = E.P;
     Derived &G = E.G;
     G.dt = p.dt;
@@ -400,11 +402,16 @@ void derive(Part &E, const xmd_topology *t) {
     for (double r : ss_rmax) mfmd = std::max(mfmd, r);
     double mc = 0.0;
     if (p.pauli_enabled) mc = std::max(mc, p.pauli_r_excl);
-    if (p.go_enabled)
-        for (int k = 0; k < t->n_native; ++k) mc = std::max(mc, 2.0 * t->nat_dist[k]);
+    if (p.go_enabled) mc = std::max(mc, go_cut);
     if (p.const_es_enabled || p.rel_es_enabled) mc = std::max(mc, es_cut);
     if (p.qa_enabled) mc = std::max(mc, mfmd);
     if (p.pid_enabled) mc = std::max(mc, pid_cut);
+    // native_ss.cpp: harmonic::cutoff(equilibrium distance) = 2 * it
+    G.ss_on = p.nat_ss_enabled;
+    G.ss_H1 = p.nat_ss_H1;
+    G.ss_equil = p.nat_ss_equil_dist;
+    G.ss_cut = 2.0 * p.nat_ss_equil_dist;
+    if (p.nat_ss_enabled) mc = std::max(mc, G.ss_cut);
     if (p.max_cutoff_override > 0) mc = p.max_cutoff_override;
     G.pad = p.pad_factor * mc;
     G.req_r = mc + G.pad;
@@ -476,12 +483,13 @@ void derive(Part &E, const xmd_topology *t) {
     else if (p.rel_es_enabled) m |= 1u << T_REL_ES;
     if (p.qa_enabled) m |= 1u << T_QA;
     if (p.pid_enabled) m |= 1u << T_PID;
+    if (p.nat_ss_enabled) m |= 1u << T_NAT_SS;
     E.mask = m;
     E.TTd.mask = m;
     E.TTf.mask = m;
 }
 
-void build(Part &E, const xmd_params *p, const xmd_topology *t) {
+void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     if (!p || !t) throw Fail{XMD_ERR_ARG, "null params or topology"};
     E.P = *p;
     if (!(p->dt > 0)) throw Fail{XMD_ERR_ARG, "dt must be positive"};
@@ -555,7 +563,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
             throw Fail{XMD_ERR_ARG, "chain spans replicas"};
         bfl[s] = f;
     }
-    derive(E, t);
+    derive(E, go_cut);
     const xmd_params &P = E.P;
     // per-replica arrays
     std::vector<Box> boxes(E.nrep);
@@ -609,6 +617,25 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
         }
     }
     nat_start[n] = (int)nat_j.size();
+    // disulfide bonds: rows of both ends, sorted; never a native contact
+    std::vector<std::vector<int>> srow(n);
+    for (int c = 0; c < t->n_ssbonds; ++c) {
+        int a = t->ss_i1[c], b = t->ss_i2[c];
+        if (a < 0 || b >= E.nbeads || !(a < b)) throw Fail{XMD_ERR_ARG, "ssbond needs 0 <= i1 < i2 < n"};
+        int sa = E.slot_of_bead[a], sb = E.slot_of_bead[b];
+        if (E.slot_rep_h[sa] != E.slot_rep_h[sb]) throw Fail{XMD_ERR_ARG, "ssbond crosses replicas"};
+        for (auto &e : nrow[sa])
+            if (e.first == sb) throw Fail{XMD_ERR_ARG, "pair is both a native contact and an ssbond"};
+        srow[sa].push_back(sb);
+        srow[sb].push_back(sa);
+    }
+    std::vector<int> ss_start(n + 1, 0), ss_j;
+    for (int s = 0; s < n; ++s) {
+        std::sort(srow[s].begin(), srow[s].end());
+        ss_start[s] = (int)ss_j.size();
+        ss_j.insert(ss_j.end(), srow[s].begin(), srow[s].end());
+    }
+    ss_start[n] = (int)ss_j.size();
     // bonded natives in slot space
     std::vector<double> teth(n, P.tether_def_length), th(n, NAN), ph(n, NAN), ch(n, NAN), cf(n, NAN);
     for (int g = 0; g < E.nbeads; ++g) {
@@ -731,6 +758,9 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t) {
     D.nat_j = E.upload(nat_j);
     D.nat_d = E.upload(nat_d);
     D.nat_act = E.alloc<uint8_t>(nat_j.size());
+    D.ss_start = E.upload(ss_start);
+    D.ss_j = E.upload(ss_j);
+    D.ss_act = E.alloc<uint8_t>(ss_j.size());
     D.sync = E.upload(sync);
     D.qslot = E.alloc<int>((size_t)MAXQ * n);
     D.qn = E.alloc<int>(n);
@@ -992,7 +1022,8 @@ static xmd_status part_get_list(Part *e, int32_t rep, int32_t id, int32_t *i1, i
         int s0 = e->rep_slot_off[rep], nb = e->rep_n[rep], n = e->nslot;
         std::vector<std::pair<int, int>> out;
         auto local = [&](int s) { return s - s0; };
-        if (id == XMD_LIST_NL_ALL || id == XMD_LIST_NL_NATIVE || id == XMD_LIST_NL_NON_NATIVE) {
+        if (id == XMD_LIST_NL_ALL || id == XMD_LIST_NL_NATIVE || id == XMD_LIST_NL_NON_NATIVE ||
+            id == XMD_LIST_NL_NAT_SS) {
             // scratch of this call only, freed on return
             Scratch<int> cnt(e, n);
             k_nl_export<<<e->nblk(), BLK, 0, e->stream>>>(e->D, e->G, cnt.p, nullptr, nullptr, nullptr, nullptr, 0);
@@ -1010,13 +1041,21 @@ static xmd_status part_get_list(Part *e, int32_t rep, int32_t id, int32_t *i1, i
             auto nat = e->get(onat, oall[n]);
             for (int s = s0; s < s0 + nb; ++s)
                 for (int k = oall[s]; k < oall[s + 1]; ++k) {
-                    bool keep = id == XMD_LIST_NL_ALL || (id == XMD_LIST_NL_NATIVE ? nat[k] : !nat[k]);
+                    bool keep = id == XMD_LIST_NL_ALL || (id == XMD_LIST_NL_NATIVE && nat[k] == 1) ||
+                                (id == XMD_LIST_NL_NON_NATIVE && nat[k] == 0) || (id == XMD_LIST_NL_NAT_SS && nat[k] == 2);
                     if (keep) out.push_back({local(a[k]), local(b[k])});
                 }
         } else if (id == XMD_LIST_GO) {
             auto st = e->get(e->D.nat_start, n + 1);
             auto j = e->get(e->D.nat_j, st[n]);
             auto act = e->get(e->D.nat_act, st[n]);
+            for (int s = s0; s < s0 + nb; ++s)
+                for (int k = st[s]; k < st[s + 1]; ++k)
+                    if (j[k] > s && act[k]) out.push_back({local(s), local(j[k])});
+        } else if (id == XMD_LIST_NAT_SS) {
+            auto st = e->get(e->D.ss_start, n + 1);
+            auto j = e->get(e->D.ss_j, st[n]);
+            auto act = e->get(e->D.ss_act, st[n]);
             for (int s = s0; s < s0 + nb; ++s)
                 for (int k = st[s]; k < st[s + 1]; ++k)
                     if (j[k] > s && act[k]) out.push_back({local(s), local(j[k])});
@@ -1317,7 +1356,7 @@ xmd_status on_part(xmd_engine *e, Part *q, xmd_status st) {
 // bead; `store` keeps the arrays it points into alive.
 struct SubTopo {
     xmd_topology t{};
-    std::vector<int32_t> rep_off, chain_off, i1, i2;
+    std::vector<int32_t> rep_off, chain_off, i1, i2, ss1, ss2;
     std::vector<double> r0, dist;
 };
 void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
@@ -1335,6 +1374,12 @@ void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
             o.i1.push_back(t->nat_i1[k] - b0);
             o.i2.push_back(t->nat_i2[k] - b0);
             o.dist.push_back(t->nat_dist[k]);
+        }
+    for (int k = 0; k < t->n_ssbonds; ++k)
+        if (t->ss_i1[k] >= b0 && t->ss_i1[k] < b1) {
+            if (t->ss_i2[k] < b0 || t->ss_i2[k] >= b1) throw Fail{XMD_ERR_ARG, "ssbond spans replicas"};
+            o.ss1.push_back(t->ss_i1[k] - b0);
+            o.ss2.push_back(t->ss_i2[k] - b0);
         }
     auto at = [&](const double *a) { return a ? a + b0 : nullptr; };
     o.t.n_replicas = r1 - r0;
@@ -1356,6 +1401,9 @@ void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
     o.t.nat_phi = at(t->nat_phi);
     o.t.nat_chir = at(t->nat_chir);
     o.t.nat_chir_factor = at(t->nat_chir_factor);
+    o.t.n_ssbonds = (int32_t)o.ss1.size();
+    o.t.ss_i1 = o.ss1.data();
+    o.t.ss_i2 = o.ss2.data();
 }
 
 void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
@@ -1365,8 +1413,12 @@ void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
     if (t->replica_offset[0] != 0 || t->replica_offset[t->n_replicas] != t->n_beads)
         throw Fail{XMD_ERR_ARG, "replica_offset must span [0, n_beads]"};
     if (p->stream_groups < 0) throw Fail{XMD_ERR_ARG, "stream_groups must be >= 0"};
+    if (t->n_ssbonds < 0 || (t->n_ssbonds > 0 && (!t->ss_i1 || !t->ss_i2))) throw Fail{XMD_ERR_ARG, "bad ssbond arrays"};
     const int R = t->n_replicas;
     const int G = std::min(R, p->stream_groups > 0 ? p->stream_groups : (R >= 2 ? 2 : 1));
+    // go.cpp cutoff: lj::cutoff(nat_dist) = 2 nat_dist, max over all contacts
+    double go_cut = 0.0;
+    for (int k = 0; k < t->n_native; ++k) go_cut = std::max(go_cut, 2.0 * t->nat_dist[k]);
     // cut points: the first replica starting at or after g/G of the beads
     std::vector<int> cut{0};
     for (int g = 1; g < G; ++g) {
@@ -1385,7 +1437,7 @@ void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
         sub_topology(t, cut[g], cut[g + 1], st);
         Part *q = new Part();
         E.parts.push_back(q);
-        build(*q, p, &st.t);
+        build(*q, p, &st.t, go_cut);
         E.part_rep0.push_back(cut[g]);
         E.part_bead0.push_back(t->replica_offset[cut[g]]);
         for (int r = cut[g]; r < cut[g + 1]; ++r) {

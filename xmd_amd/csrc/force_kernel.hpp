@@ -134,6 +134,18 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
                 if (s < j) addV(T_GO, (double)f.V);
             }
         }
+        // disulfide bonds (native_ss.cpp:27-40), FP64 in both forms
+        if (mask & (1u << T_NAT_SS)) {
+            for (int k = D.ss_start[s] + sub; k < D.ss_start[s + 1]; k += LPB) {
+                if (!D.ss_act[k]) continue;
+                const int j = D.ss_j[k];
+                const v3<double> d = wrap(box, ldp(Pp, j) - ri);
+                const double r = norm(d);
+                const VdV<double> e = harmonic<double>(G.ss_H1, 0.0, G.ss_equil, r);
+                add3(Fx, Fy, Fz, d * (e.dV / r));
+                if (s < j) addV(T_NAT_SS, e.V);
+            }
+        }
         // neighbour row: Pauli, Debye-Hueckel, PID
         uint32_t rmask = mask & ((1u << T_PAULI) | (1u << T_CONST_ES) | (1u << T_REL_ES) | (1u << T_PID));
         if (rmask && !(D.err[0] & E_NL_CAP)) {

@@ -626,6 +626,9 @@ __device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &
     v3<double> d = wrap(box, ldp(P, j) - rs);
     double dist = norm(d);
     if (!(dist < G.req_r)) return 0;
+    // disulfide pair: in native_ssbonds only, no pair term (D24)
+    for (int k = D.ss_start[s]; k < D.ss_start[s + 1]; ++k)
+        if (D.ss_j[k] == j) return F_NATIVE | F_NATSS;
     // native?  rows are short and sorted
     bool native = false;
     for (int k = D.nat_start[s]; k < D.nat_start[s + 1]; ++k)
@@ -796,6 +799,11 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
         double cutoff = 2.0 * D.nat_d[k];
         D.nat_act[k] = norm(d) < cutoff + G.pad;
     }
+    // current disulfide bonds (native_ss.cpp:8-19, D23)
+    for (int k = D.ss_start[s] + l; k < D.ss_start[s + 1]; k += WAVE) {
+        v3<double> d = wrap(box, ldp(P, D.ss_j[k]) - rs);
+        D.ss_act[k] = norm(d) < G.ss_cut + G.pad;
+    }
 }
 
 __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double *P, int p) {
@@ -964,12 +972,14 @@ __global__ __launch_bounds__(BLK) void k_nl_export(Dev D, Derived G, int *cnt, c
         v3<double> d = wrap(box, ldp(D.orig_r, j) - rs);
         if (norm(d) < G.req_r) {
             if (mode) {
-                bool native = false;
+                uint8_t kind = 0;  // 0 non_native, 1 native_contacts, 2 native_ssbonds
                 for (int q = D.nat_start[s]; q < D.nat_start[s + 1]; ++q)
-                    if (D.nat_j[q] == j) native = true;
+                    if (D.nat_j[q] == j) kind = 1;
+                for (int q = D.ss_start[s]; q < D.ss_start[s + 1]; ++q)
+                    if (D.ss_j[q] == j) kind = 2;
                 oi1[k] = s;
                 oi2[k] = j;
-                onat[k] = native;
+                onat[k] = kind;
             }
             ++k;
         }

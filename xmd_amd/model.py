@@ -405,9 +405,9 @@ def morph_into_saw(m, gen, bond_length, residue_density, infer_box):
 
 # ------------------------------------------------------- model -> engine ---
 def to_system(m, temperature, rng_state):
-    """One-replica engine topology of a model (main.cpp:113-206,372-553):
+    """One-replica engine topology of a model (main.cpp:113-206,372-577):
     positions, types, chains; Go contacts = contacts other than NAT_SS at
-    their lengths; per-bead native tether length, angle and dihedral where
+    their lengths; disulfide bonds = the NAT_SS contacts; per-bead native tether length, angle and dihedral where
     the model gives a value (angles and dihedrals must join consecutive
     residues of one chain); chirality of every model dihedral from the
     model's positions (after morphing, as the reference sets it up)."""
@@ -457,6 +457,7 @@ def to_system(m, temperature, rng_state):
     nat_i1 = np.array([a for a, _, _ in nat], dtype=np.int32)
     nat_i2 = np.array([b for _, b, _ in nat], dtype=np.int32)
     nat_d = np.array([d for _, _, d in nat], dtype=np.float64)
+    ss = sorted((min(slot[i], slot[j]), max(slot[i], slot[j])) for i, j, _, t in m.contacts if t == NAT_SS)
 
     return System(
         replica_offset=np.array([0, n], dtype=np.int32),
@@ -468,6 +469,7 @@ def to_system(m, temperature, rng_state):
         rng_state=np.array([int(rng_state) & RandGen.MASK], dtype=np.uint64),
         nat_i1=nat_i1, nat_i2=nat_i2, nat_dist=nat_d,
         tether_len=tether, nat_theta=theta, nat_phi=phi, nat_chir=chir, nat_chir_factor=chir_f,
+        ss_i1=np.array([a for a, _ in ss], dtype=np.int32), ss_i2=np.array([b for _, b in ss], dtype=np.int32),
     )
 
 

@@ -244,6 +244,9 @@ def to_xmd_params(tree, base_dir=".", precision=_abi.PREC_FP64, rng_exact=True,
     p.pauli_enabled, p.pauli_r_excl, p.pauli_depth = int(_bool(pa["enabled"])), q(pa["r_excl"]), q(pa["depth"])
     g = tree["native contacts"]
     p.go_enabled, p.go_depth = int(_bool(g["enabled"])), q(g["lj depth"])
+    ss = tree["native ssbonds"]
+    p.nat_ss_enabled, p.nat_ss_H1 = int(_bool(ss["enabled"])), q(ss["H1"])
+    p.nat_ss_equil_dist = q(ss["equilibrium dist"])
     e = tree["const ES"]
     p.const_es_enabled = int(_bool(e["enabled"]))
     p.const_es_screen, p.const_es_permittivity = q(e["screening distance"]), q(e["permittivity"])

@@ -119,6 +119,8 @@ class System:
     nat_phi: np.ndarray
     nat_chir: np.ndarray
     nat_chir_factor: np.ndarray
+    ss_i1: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    ss_i2: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -150,14 +152,17 @@ class System:
             nat_phi=np.ascontiguousarray(self.nat_phi, dtype=np.float64),
             nat_chir=np.ascontiguousarray(self.nat_chir, dtype=np.float64),
             nat_chir_factor=np.ascontiguousarray(self.nat_chir_factor, dtype=np.float64),
+            ss_i1=np.ascontiguousarray(self.ss_i1, dtype=np.int32),
+            ss_i2=np.ascontiguousarray(self.ss_i2, dtype=np.int32),
         )
         t = _abi.XmdTopology()
         t.n_replicas = self.n_replicas
         t.n_beads = self.n_beads
         t.n_chains = len(self.chain_offset) - 1
         t.n_native = len(self.nat_i1)
+        t.n_ssbonds = len(self.ss_i1)
         types = dict(replica_offset=C.c_int32, chain_offset=C.c_int32, atype=C.c_int8,
-                     rng_state=C.c_uint64, nat_i1=C.c_int32, nat_i2=C.c_int32)
+                     rng_state=C.c_uint64, nat_i1=C.c_int32, nat_i2=C.c_int32, ss_i1=C.c_int32, ss_i2=C.c_int32)
         for name, a in arrs.items():
             setattr(t, name, _abi.ptr(a, types.get(name, C.c_double)))
         t._arrays = arrs  # keep alive with the struct
@@ -246,15 +251,15 @@ def random_chain(n, rng):
 
 # Term sets of the configurations (names of the YAML sections' enabled flags).
 C0_TERMS = dict(go=True, pauli=True, tether=True, chir=True, nat_ang=False, cnd=False, snd=False,
-                const_es=False, rel_es=False, qa=False, pid=False)
+                const_es=False, rel_es=False, qa=False, pid=False, nat_ss=False)
 C1_TERMS = dict(C0_TERMS, const_es=True, nat_ang=True, cnd=True)
 FULL_TERMS = dict(go=True, pauli=True, tether=True, chir=True, nat_ang=True, cnd=True, snd=False,
-                  const_es=True, rel_es=False, qa=True, pid=True)
+                  const_es=True, rel_es=False, qa=True, pid=True, nat_ss=False)
 C3_TERMS = dict(C0_TERMS, const_es=True)
 
 _FLAG = dict(go="go_enabled", pauli="pauli_enabled", tether="tether_enabled", chir="chir_enabled",
              nat_ang="nat_ang_enabled", cnd="cnd_enabled", snd="snd_enabled", const_es="const_es_enabled",
-             rel_es="rel_es_enabled", qa="qa_enabled", pid="pid_enabled")
+             rel_es="rel_es_enabled", qa="qa_enabled", pid="pid_enabled", nat_ss="nat_ss_enabled")
 
 
 def apply_terms(p, terms):
@@ -356,7 +361,8 @@ def replicate(system, temperatures, seeds):
         rng_state=np.asarray(seeds, dtype=np.uint64),
         nat_i1=shifted(system.nat_i1), nat_i2=shifted(system.nat_i2), nat_dist=tile(system.nat_dist),
         tether_len=tile(system.tether_len), nat_theta=tile(system.nat_theta), nat_phi=tile(system.nat_phi),
-        nat_chir=tile(system.nat_chir), nat_chir_factor=tile(system.nat_chir_factor))
+        nat_chir=tile(system.nat_chir), nat_chir_factor=tile(system.nat_chir_factor),
+        ss_i1=shifted(system.ss_i1).astype(np.int32), ss_i2=shifted(system.ss_i2).astype(np.int32))
 
 
 def concat(systems):
@@ -374,7 +380,8 @@ def concat(systems):
         rng_state=cat("rng_state").astype(np.uint64),
         nat_i1=shift("nat_i1").astype(np.int32), nat_i2=shift("nat_i2").astype(np.int32), nat_dist=cat("nat_dist"),
         tether_len=cat("tether_len"), nat_theta=cat("nat_theta"), nat_phi=cat("nat_phi"),
-        nat_chir=cat("nat_chir"), nat_chir_factor=cat("nat_chir_factor"))
+        nat_chir=cat("nat_chir"), nat_chir_factor=cat("nat_chir_factor"),
+        ss_i1=shift("ss_i1").astype(np.int32), ss_i2=shift("ss_i2").astype(np.int32))
 
 
 def perturbed_positions(system, seed, sigma_A=0.3):
