@@ -82,10 +82,13 @@ def test_model_mixed_precision(glut):
     params.precision = _abi.PREC_FP64
     ref = Engine(params, system)
     eng.step(50)
+    # the same state in both: Gear derivatives, generator, time, QA contacts
     for r in range(system.n_replicas):
-        y, *_ = eng.get_state(r)
-        ref.set_positions(r, y[0])
-        eng.set_positions(r, y[0])
+        y, _, rng, t = eng.get_state(r)
+        cont, sync = eng.get_qa_state(r)
+        for e in (ref, eng):
+            e.set_state(r, y, rng, t)
+            e.set_qa_state(r, cont, sync)
     F, _ = eng.eval_forces()
     Fr, _ = ref.eval_forces()
     assert rel_rms(F, Fr) < 1e-5, rel_rms(F, Fr)
