@@ -62,7 +62,11 @@ typedef enum xmd_term {
     XMD_TERM_QA = 9,        /* "quasi-adiabatic potential" src/forces/qa/process_contacts.cpp:13-57 */
     XMD_TERM_PID = 10,      /* "pseudo-improper dihedral potential" src/forces/pid/eval_forces.cpp:14-141 */
     XMD_TERM_NAT_SS = 11,   /* "native ssbonds"       src/forces/native_ss.cpp:27-40         */
-    XMD_NUM_TERMS = 12
+    XMD_TERM_SOLID_WALL = 12, /* "solid walls"        src/forces/walls/solid.cpp:19-30       */
+    XMD_TERM_LJ_WALL = 13,  /* "LJ attractive walls"  src/forces/walls/lj_attr.cpp:7-49      */
+    XMD_TERM_VEL_AFM = 14,  /* "velocity AFM"         src/forces/velocity_afm.cpp:12-24      */
+    XMD_TERM_FORCE_AFM = 15,/* "force AFM"            src/forces/force_afm.cpp:11-14         */
+    XMD_NUM_TERMS = 16
 } xmd_term;
 
 /* Lists produced by a Verlet rebuild (xmd/include/xmd/nl/data.h:8-14 and the
@@ -159,6 +163,14 @@ typedef struct xmd_params {
     /* "native ssbonds" (params.cpp:180-184) */
     int32_t nat_ss_enabled;
     double nat_ss_H1, nat_ss_equil_dist;
+    /* "solid walls", "LJ attractive walls" (params.cpp:223-246; planes in
+       xmd_topology) */
+    int32_t solid_walls_enabled, lj_walls_enabled;
+    double solid_walls_depth, solid_walls_cutoff;
+    double lj_walls_min_dist, lj_walls_breaking_factor, lj_walls_phase_duration;
+    /* "velocity AFM", "force AFM" (params.cpp:248-270; tips in xmd_topology) */
+    int32_t vel_afm_enabled, force_afm_enabled;
+    double vel_afm_H1, vel_afm_H2;
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and
@@ -190,6 +202,22 @@ typedef struct xmd_topology {
     int32_t n_ssbonds;
     const int32_t *ss_i1;
     const int32_t *ss_i2;
+    /* wall planes, the same for every replica: 6 doubles each (origin x, y,
+       z, unit normal x, y, z) */
+    int32_t n_solid_walls;
+    const double *solid_walls;
+    int32_t n_lj_walls;
+    const double *lj_walls;
+    /* AFM tips, global bead indices: velocity tips pull bead idx towards
+       orig + t v (orig NaN = the bead's initial position), force tips add a
+       constant force */
+    int32_t n_vel_afm;
+    const int32_t *vel_afm_idx;
+    const double *vel_afm_orig;   /* [3*n_vel_afm] */
+    const double *vel_afm_v;      /* [3*n_vel_afm] */
+    int32_t n_force_afm;
+    const int32_t *force_afm_idx;
+    const double *force_afm_F;    /* [3*n_force_afm] */
 } xmd_topology;
 
 /* QA contact record (xmd/include/xmd/forces/qa/contact.h:15-16). */
@@ -259,6 +287,14 @@ xmd_status xmd_get_qa_state(xmd_engine *eng, int32_t replica,
 xmd_status xmd_set_qa_state(xmd_engine *eng, int32_t replica,
                             const xmd_qa_contact *contacts, int64_t n_contacts,
                             const int8_t *sync);
+
+/* State of the LJ attractive walls of one replica (the reference's saved
+ * "lj_attr.pairs", main.cpp:850-860): per bead b and wall w, at index
+ * b * n_lj_walls + w, status (0 FREE, 1 FORMING_OR_FORMED, 2 BREAKING),
+ * joint point (3 doubles) and reference time. */
+xmd_status xmd_get_wall_state(xmd_engine *eng, int32_t replica, int8_t *status, double *joint, double *ref_t);
+xmd_status xmd_set_wall_state(xmd_engine *eng, int32_t replica, const int8_t *status, const double *joint,
+                              const double *ref_t);
 
 /* Run nsteps of the loop body (main.cpp:1028-1076 without reporters) on
  * every replica: forces, Langevin predictor-corrector, verify, rebuild. */

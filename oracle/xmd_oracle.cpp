@@ -52,6 +52,17 @@
 //       (harmonic::cutoff, the value main.cpp:592 adds to max_cutoff) plus
 //       the list pad; update_nat_ssbonds::cutoff is never set
 //       (native_ss.h:44, main.cpp:587-591).
+//   D25 solid walls: V = eps / (9 x^9) as written, force -dV/dx n
+//       (walls/solid.cpp:24-25 writes dV/dx = V (-9 x) and adds it with
+//       the opposite sign).
+//   D26 LJ attractive walls: depth 1 eps (lj_attr.h:31 lj_depth is never
+//       set and has no parameter); the pair state changes during the
+//       step's force pass, as in lj_attr.cpp:14-44.
+//   D27 velocity AFM: the harmonic's rest length is 0 (velocity_afm.h:21
+//       default-constructs it; main.cpp sets H1, H2 only); a tip without
+//       an origin starts at its bead's initial position (main.cpp:884-886
+//       reads an undefined pulled_idx).  Neither AFM term adds energy
+//       (velocity_afm.cpp:21, force_afm.cpp:13).
 //   D24 disulfide pairs of the list (native_ssbonds) belong to neither
 //       native_contacts nor non_native, so no pair term but the bond sees
 //       them (Debye-Hueckel reads native_contacts + non_native, D2).
@@ -328,6 +339,9 @@ template <typename R> struct Params {
     R pad_factor, max_cutoff;
     bool tether_on, chir_on, ang_on, cnd_on, snd_on, pauli_on, go_on, ces_on, res_on, qa_on, pid_on, ss_on;
     R ss_H1, ss_equil;
+    bool solid_on, ljw_on, vafm_on, fafm_on;
+    R solid_eps, solid_cut, ljw_min_dist, ljw_factor, ljw_cycle, ljw_cycle_inv, vafm_H1, vafm_H2;
+    std::vector<std::array<R, 6>> solid_walls, lj_walls;  // origin, normal
     R H1, H2, def_len, e_chi, ang_k, CDA, CDB, CDH;
     R pauli_r_excl, pauli_depth, go_depth;
     R ces_Vf, ces_sdi, res_Vf, res_sdi, es_cutoff;
@@ -374,6 +388,22 @@ template <typename R> struct Replica {
     std::vector<NatCont<R>> all_nat, go_cur;
     std::vector<std::pair<int, int>> all_nat_nl;  // sorted (i1,i2)
     std::vector<std::pair<int, int>> all_ss, ss_cur;  // sorted (i1,i2); current
+    struct LjWall {
+        int8_t status = 0;  // 0 FREE, 1 FORMING_OR_FORMED, 2 BREAKING
+        V3<R> joint{0, 0, 0};
+        R ref_t = 0;
+    };
+    std::vector<LjWall> ljw;  // bead * n_lj_walls + wall
+    struct VelTip {
+        int i;
+        V3<R> orig, v;
+    };
+    struct ForceTip {
+        int i;
+        V3<R> F;
+    };
+    std::vector<VelTip> vtips;
+    std::vector<ForceTip> ftips;
     std::vector<std::pair<int, int>> pauli;
     std::vector<EsPair<R>> es;
     std::vector<PidBundle> pid;
@@ -528,6 +558,79 @@ template <typename R> void eval_nat_ss(const Params<R> &P, Replica<R> &s, std::v
         F[i1] += dV_dr * r12_u;
         F[i2] -= dV_dr * r12_u;
     }
+}
+// xmd/src/forces/walls/solid.cpp:19-30 (+D25)
+template <typename R> void eval_solid_walls(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, size_t lo = 0,
+                                            size_t hi = SIZE_MAX) {
+    for (size_t i = lo; i < std::min(hi, (size_t)s.n); ++i) {
+        V3<R> pr = s.box.wrap(s.r[i]);
+        for (auto const &w : P.solid_walls) {
+            V3<R> o{w[0], w[1], w[2]}, nn{w[3], w[4], w[5]};
+            R x = dot(nn, pr - o);
+            if (x < P.solid_cut) {
+                R x3 = x * x * x;
+                R V_ = P.solid_eps / (R(9.0) * (x3 * x3 * x3));
+                R dV_dx = -R(9.0) * V_ / x;
+                *V += V_;
+                F[i] -= dV_dx * nn;
+            }
+        }
+    }
+}
+// xmd/src/forces/walls/lj_attr.cpp:7-49 (+D26); mutate: the step's pass
+template <typename R> void eval_lj_walls(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, bool mutate,
+                                         size_t lo = 0, size_t hi = SIZE_MAX) {
+    const size_t nw = P.lj_walls.size();
+    for (size_t k = lo; k < std::min(hi, s.ljw.size()); ++k) {
+        auto &pr = s.ljw[k];
+        const int i = (int)(k / nw);
+        auto const &w = P.lj_walls[k % nw];
+        V3<R> o{w[0], w[1], w[2]}, nn{w[3], w[4], w[5]};
+        V3<R> r_ = s.box.wrap(s.r[i]);
+        if (pr.status == 0) {
+            R x = dot(nn, r_ - o);
+            if (std::abs(x) < P.ljw_min_dist && mutate) {
+                pr.status = 1;
+                pr.joint = r_ - nn * x;
+                pr.ref_t = s.t;
+            }
+            continue;
+        }
+        R saturation = std::min(s.t - pr.ref_t, P.ljw_cycle) * P.ljw_cycle_inv;
+        if (pr.status == 2) saturation = R(1.0) - saturation;
+        V3<R> r12 = pr.joint - r_;
+        R r12_rn = norm_inv(r12);
+        auto [V_, dV_dr] = LJ<R>{R(1.0), P.ljw_min_dist}(r12_rn);
+        *V += V_ * saturation;
+        V3<R> r12_u = r12 * r12_rn;
+        F[i] += saturation * dV_dr * r12_u;
+        if (!mutate) continue;
+        if (pr.status == 1 && saturation == R(1.0f)) {
+            if (P.ljw_factor * P.ljw_min_dist * r12_rn < R(1.0f)) {
+                pr.status = 2;
+                pr.ref_t = s.t;
+            }
+        } else if (pr.status == 2 && saturation == R(0.0f)) {
+            pr.status = 0;
+        }
+    }
+}
+// xmd/src/forces/velocity_afm.cpp:12-24, force_afm.cpp:11-14 (+D27)
+template <typename R> void eval_vel_afm(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, size_t lo = 0,
+                                        size_t hi = SIZE_MAX) {
+    for (size_t k = lo; k < std::min(hi, s.vtips.size()); ++k) {
+        auto const &tip = s.vtips[k];
+        V3<R> cur = tip.orig + s.t * tip.v;
+        V3<R> r_afm = cur - s.r[tip.i];
+        R r_afm_n = norm(r_afm), r_afm_rn = R(1.0f) / r_afm_n;
+        V3<R> r_afm_u = r_afm * r_afm_rn;
+        auto [V_, dV_dr] = Harmonic<R>{P.vafm_H1, P.vafm_H2, R(0.0)}(r_afm_n);
+        (void)V_;
+        F[tip.i] += dV_dr * r_afm_u;
+    }
+}
+template <typename R> void eval_force_afm(Replica<R> &s, std::vector<V3<R>> &F, size_t lo = 0, size_t hi = SIZE_MAX) {
+    for (size_t k = lo; k < std::min(hi, s.ftips.size()); ++k) F[s.ftips[k].i] += s.ftips[k].F;
 }
 template <typename R> void eval_es(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, bool relative, size_t lo = 0,
              size_t hi = SIZE_MAX) {
@@ -915,6 +1018,11 @@ template <typename R> struct System : SysBase {
             qa_process_contacts(P, s, F, V(XMD_TERM_QA), qa_mutate);
         }
         if (P.pid_on && on(XMD_TERM_PID)) eval_pid(P, s, F, V(XMD_TERM_PID));
+        // main.cpp:987-1005
+        if (P.solid_on && on(XMD_TERM_SOLID_WALL)) eval_solid_walls(P, s, F, V(XMD_TERM_SOLID_WALL));
+        if (P.ljw_on && on(XMD_TERM_LJ_WALL)) eval_lj_walls(P, s, F, V(XMD_TERM_LJ_WALL), qa_mutate);
+        if (P.vafm_on && on(XMD_TERM_VEL_AFM)) eval_vel_afm(P, s, F);
+        if (P.fafm_on && on(XMD_TERM_FORCE_AFM)) eval_force_afm(s, F);
     }
     void qa_bookkeeping_pre(Replica<R> &s) {
         qa_precompute_nh(s);
@@ -1020,6 +1128,10 @@ template <typename R> struct System : SysBase {
 #pragma omp barrier
         }
         if (P.pid_on) { chunk(s.pid.size(), lo, hi); range_eval(s, &Replica<R>::pid, lo, hi, F, V, 6); }
+        if (P.solid_on) { chunk((size_t)s.n, lo, hi); if (lo < hi) eval_solid_walls(P, s, F, V, lo, hi); }
+        if (P.ljw_on) { chunk(s.ljw.size(), lo, hi); if (lo < hi) eval_lj_walls(P, s, F, V, true, lo, hi); }
+        if (P.vafm_on) { chunk(s.vtips.size(), lo, hi); if (lo < hi) eval_vel_afm(P, s, F, lo, hi); }
+        if (P.fafm_on) { chunk(s.ftips.size(), lo, hi); if (lo < hi) eval_force_afm(s, F, lo, hi); }
     }
     // Contact state transitions of process_contacts, serially, after the
     // parallel force pass (forces of the step are unaffected).
@@ -1072,6 +1184,18 @@ template <typename R> int setup_params(const xmd_params *p, Params<R> &P) {
     P.qa_on = p->qa_enabled;
     P.pid_on = p->pid_enabled;
     P.ss_on = p->nat_ss_enabled;
+    P.solid_on = p->solid_walls_enabled;
+    P.ljw_on = p->lj_walls_enabled;
+    P.vafm_on = p->vel_afm_enabled;
+    P.fafm_on = p->force_afm_enabled;
+    P.solid_eps = (R)p->solid_walls_depth;
+    P.solid_cut = (R)p->solid_walls_cutoff;
+    P.ljw_min_dist = (R)p->lj_walls_min_dist;
+    P.ljw_factor = (R)(p->lj_walls_breaking_factor * std::pow(2.0, -1.0 / 6.0));  // main.cpp:840
+    P.ljw_cycle = (R)p->lj_walls_phase_duration;
+    P.ljw_cycle_inv = (R)(1.0 / (double)P.ljw_cycle);
+    P.vafm_H1 = (R)p->vel_afm_H1;
+    P.vafm_H2 = (R)p->vel_afm_H2;
     P.ss_H1 = (R)p->nat_ss_H1;
     P.ss_equil = (R)p->nat_ss_equil_dist;
     P.H1 = (R)p->tether_H1;
@@ -1158,6 +1282,16 @@ template <typename R> int build_system(const xmd_params *p, const xmd_topology *
     if (p->max_cutoff_override > 0) mc = (R)p->max_cutoff_override;
     P.max_cutoff = mc;
 
+    for (int w = 0; w < t->n_solid_walls; ++w) {
+        std::array<R, 6> a;
+        for (int k = 0; k < 6; ++k) a[k] = (R)t->solid_walls[6 * w + k];
+        P.solid_walls.push_back(a);
+    }
+    for (int w = 0; w < t->n_lj_walls; ++w) {
+        std::array<R, 6> a;
+        for (int k = 0; k < 6; ++k) a[k] = (R)t->lj_walls[6 * w + k];
+        P.lj_walls.push_back(a);
+    }
     S.n_rep = t->n_replicas;
     S.n_total = t->n_beads;
     S.rep_off.assign(t->replica_offset, t->replica_offset + t->n_replicas + 1);
@@ -1200,6 +1334,21 @@ template <typename R> int build_system(const xmd_params *p, const xmd_topology *
         }
         s.box.set_cell({(R)t->box[3 * k], (R)t->box[3 * k + 1], (R)t->box[3 * k + 2]});
         s.temperature = (R)t->temperature[k];
+        // walls (the same planes in every replica) and AFM tips of this replica
+        s.ljw.assign((size_t)n * P.lj_walls.size(), {});
+        for (int c = 0; c < t->n_vel_afm; ++c) {
+            int g = t->vel_afm_idx[c];
+            if (g < off || g >= off + n) continue;
+            const double *o = t->vel_afm_orig + 3 * c, *v = t->vel_afm_v + 3 * c;
+            V3<R> orig = std::isnan(o[0]) ? s.r[g - off] : V3<R>{(R)o[0], (R)o[1], (R)o[2]};
+            s.vtips.push_back({g - off, orig, {(R)v[0], (R)v[1], (R)v[2]}});
+        }
+        for (int c = 0; c < t->n_force_afm; ++c) {
+            int g = t->force_afm_idx[c];
+            if (g < off || g >= off + n) continue;
+            const double *f = t->force_afm_F + 3 * c;
+            s.ftips.push_back({g - off, {(R)f[0], (R)f[1], (R)f[2]}});
+        }
         s.gen.state = t->rng_state[k];
         s.nvec.assign(n, {0, 0, 0});
         s.hvec.assign(n, {0, 0, 0});
@@ -1399,6 +1548,34 @@ int oracle_get_list(oracle_sys *h, int replica, int id, int32_t *i1, int32_t *i2
                 i1[k] = out[k].first;
                 i2[k] = out[k].second;
             }
+        return 0;
+    });
+}
+
+int oracle_get_wall_state(oracle_sys *h, int replica, int8_t *status, double *joint, double *ref_t) {
+    return dispatch(h, [&](auto &S) {
+        if (replica < 0 || replica >= S.n_rep) return fail("replica out of range");
+        auto &s = S.reps[replica];
+        for (size_t k = 0; k < s.ljw.size(); ++k) {
+            if (status) status[k] = s.ljw[k].status;
+            if (joint) joint[3 * k] = s.ljw[k].joint.x, joint[3 * k + 1] = s.ljw[k].joint.y, joint[3 * k + 2] = s.ljw[k].joint.z;
+            if (ref_t) ref_t[k] = s.ljw[k].ref_t;
+        }
+        return 0;
+    });
+}
+
+int oracle_set_wall_state(oracle_sys *h, int replica, const int8_t *status, const double *joint, const double *ref_t) {
+    return dispatch(h, [&](auto &S) {
+        using R = std::remove_reference_t<decltype(S.reps[0].t)>;
+        if (replica < 0 || replica >= S.n_rep) return fail("replica out of range");
+        auto &s = S.reps[replica];
+        for (size_t k = 0; k < s.ljw.size(); ++k) {
+            if (status[k] < 0 || status[k] > 2) return fail("bad wall status");
+            s.ljw[k].status = status[k];
+            s.ljw[k].joint = {(R)joint[3 * k], (R)joint[3 * k + 1], (R)joint[3 * k + 2]};
+            s.ljw[k].ref_t = (R)ref_t[k];
+        }
         return 0;
     });
 }

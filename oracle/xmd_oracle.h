@@ -56,6 +56,10 @@ uint64_t oracle_rng_next(uint64_t *state);
 float oracle_rng_uniformf(uint64_t *state);
 void oracle_rand_normals(uint64_t state, int64_t n, float *out,
                          uint64_t *state_out);
+/* LJ attractive wall state, as xmd_get_wall_state / xmd_set_wall_state. */
+int oracle_get_wall_state(oracle_sys *h, int replica, int8_t *status, double *joint, double *ref_t);
+int oracle_set_wall_state(oracle_sys *h, int replica, const int8_t *status, const double *joint,
+                          const double *ref_t);
 /* Pure primitives for closed-form tests: out = {V, dV/dr}. */
 void oracle_prim(int kind, const double *args, double *out);
 // chains of the given lengths laid out as random walks (model.cpp:92-174);

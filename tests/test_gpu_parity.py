@@ -394,3 +394,77 @@ def test_nat_ss_bonds(prec):
         Fo, Vo = ora.eval_forces(1 << k)
         assert rel_rms(F, Fo) < 1e-12 and abs(V[0, k] - Vo[0, k]) < 1e-12 * abs(Vo[0, k])
     eng.close()
+
+
+def test_checkpoint_restart_engine(tmp_path):
+    """Engine checkpoints: restart bit-identical to the saved run after the
+    same rebuild; the same checkpoint loaded into the oracle continues at
+    the FP64 trajectory bar."""
+    from xmd_amd import checkpoint
+    system = synth.config_system(2, n_batch=2, chains=4, chain_len=60)
+    params = synth.config_params(2)
+    a = Engine(params, system)
+    a.step(60)
+    path = str(tmp_path / "ckpt.npz")
+    checkpoint.save(a, path)
+    checkpoint.load(a, path)
+    b = Engine(params, system)
+    checkpoint.load(b, path)
+    ora = Oracle(params, system)
+    checkpoint.load(ora, path)
+    a.step(20)
+    b.step(20)
+    ora.step(20)
+    for r in range(2):
+        ya, _, ra, ta = a.get_state(r)
+        yb, _, rb, tb = b.get_state(r)
+        yo, _, ro, to = ora.get_state(r)
+        assert ra == rb == ro and ta == tb == to
+        assert np.array_equal(ya, yb)
+        assert np.abs(ya - yo).max() < 1e-8
+        assert np.array_equal(a.get_qa_state(r)[1], b.get_qa_state(r)[1])
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_MIXED])
+def test_walls_and_afm(prec, tmp_path):
+    """Solid and LJ attractive walls and both AFM tips against the oracle:
+    per-term forces (FP64 in both forms), 40-step trajectories with the
+    wall pairs' states bit-equal, and a checkpoint carrying them."""
+    from test_oracle_pins import walls_afm_system
+    from xmd_amd import checkpoint
+    system = walls_afm_system(synth.config_system(1, chain_len=40))
+    params = synth.config_params(1)
+    params.solid_walls_enabled = params.lj_walls_enabled = 1
+    params.vel_afm_enabled = params.force_afm_enabled = 1
+    params.lj_walls_phase_duration = 0.05
+    ora = Oracle(params, system)
+    params.precision = prec
+    eng = Engine(params, system)
+    for name in ("solid_wall", "lj_wall", "vel_afm", "force_afm"):
+        k = _abi.TERM[name]
+        F, V = eng.eval_forces(1 << k)
+        Fo, Vo = ora.eval_forces(1 << k)
+        assert rel_rms(F, Fo) < 1e-10, name
+        assert abs(V[0, k] - Vo[0, k]) <= 1e-10 * max(1.0, abs(Vo[0, k])), name
+    if prec == _abi.PREC_MIXED:
+        eng.close()
+        return
+    eng.step(40)
+    ora.step(40)
+    y, _, rng, t = eng.get_state(0)
+    yo, _, rngo, to = ora.get_state(0)
+    assert rng == rngo and t == to
+    assert np.abs(y - yo).max() < 1e-9
+    st, jo, rt = eng.get_wall_state(0)
+    sto, joo, rto = ora.get_wall_state(0)
+    assert np.array_equal(st, sto) and np.array_equal(rt, rto) and np.abs(jo - joo).max() < 1e-9
+    assert (st == 1).sum() > 0
+    path = str(tmp_path / "ckpt.npz")
+    checkpoint.save(eng, path)
+    eng2 = Engine(params, system)
+    checkpoint.load(eng2, path)
+    assert np.array_equal(eng2.get_wall_state(0)[0], st)
+    eng.close()
+    eng2.close()

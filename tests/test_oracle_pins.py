@@ -150,7 +150,8 @@ def _single_term(cid, term, **kw):
 def synth_key(term):
     return {"tether": "tether", "chiral": "chir", "nat_ang": "nat_ang", "cnd": "cnd", "snd": "snd",
             "pauli": "pauli", "go": "go", "const_es": "const_es", "rel_es": "rel_es", "qa": "qa",
-            "pid": "pid", "nat_ss": "nat_ss"}[term]
+            "pid": "pid", "nat_ss": "nat_ss", "solid_wall": "solid_wall", "lj_wall": "lj_wall",
+            "vel_afm": "vel_afm", "force_afm": "force_afm"}[term]
 
 
 @pytest.fixture(scope="module")
@@ -416,3 +417,112 @@ def test_nat_ss_bonds(small_open):
             fd = -(Vp[0, _abi.TERM["nat_ss"]] - Vm[0, _abi.TERM["nat_ss"]]) / (2 * h)
             assert F[a, i] == pytest.approx(fd, rel=1e-6, abs=1e-8)
     assert np.abs(F[:, 39]).max() == 0 and np.abs(F[:, 10]).max() == 0
+
+
+# ------------------------------------------------------- walls and AFM ---
+def walls_afm_system(base):
+    """The chain between a solid wall under it (normal +z) and an LJ
+    attractive wall beside it (normal +x), a velocity tip on bead 0 and a
+    force tip on the last bead."""
+    import dataclasses
+    r = base.r0
+    zmin, xmin = r[2].min(), r[0].min()
+    solid = np.array([[0.0, 0.0, zmin - 0.6, 0.0, 0.0, 1.0]])
+    lj = np.array([[xmin - 0.5, 0.0, 0.0, 1.0, 0.0, 0.0]])
+    n = base.n_beads
+    return dataclasses.replace(base, solid_walls=solid, lj_walls=lj,
+                               vel_afm_idx=np.array([0], dtype=np.int32),
+                               vel_afm_orig=np.array([[np.nan] * 3]), vel_afm_v=np.array([[0.0, 0.0, 0.05]]),
+                               force_afm_idx=np.array([n - 1], dtype=np.int32),
+                               force_afm_F=np.array([[0.3, -0.2, 0.1]]))
+
+
+def walls_afm_params(cid):
+    p = _single_term(cid, "tether")
+    p.solid_walls_enabled = p.lj_walls_enabled = p.vel_afm_enabled = p.force_afm_enabled = 1
+    return p
+
+
+def test_solid_wall_closed_form_and_gradient(small_open):
+    system = walls_afm_system(small_open)
+    p = _single_term(1, "solid_wall")
+    ora = Oracle(p, system)
+    r = system.r0
+    k = _abi.TERM["solid_wall"]
+    F, V = ora.eval_forces(1 << k)
+    x = r[2] - system.solid_walls[0, 2]
+    near = x < p.solid_walls_cutoff
+    assert near.sum() > 0
+    assert V[0, k] == pytest.approx((p.solid_walls_depth / (9 * x[near] ** 9)).sum(), rel=1e-13)
+    # pushes along +z only, -dV/dx = eps / x^10
+    assert np.allclose(F[2, near], p.solid_walls_depth / x[near] ** 10, rtol=1e-13)
+    assert np.abs(F[:2]).max() == 0 and np.abs(F[2, ~near]).max() == 0
+    h = 1e-6
+    i = int(np.argmin(x))
+    rp, rm = r.copy(), r.copy()
+    rp[2, i] += h
+    rm[2, i] -= h
+    ora.set_positions(0, rp); Vp = ora.eval_forces(1 << k)[1][0, k]
+    ora.set_positions(0, rm); Vm = ora.eval_forces(1 << k)[1][0, k]
+    assert F[2, i] == pytest.approx(-(Vp - Vm) / (2 * h), rel=1e-6)
+
+
+def test_lj_wall_timeline(small_open):
+    """Beads within wall min dist of the plane join it at the projection
+    of their position; the LJ toward the joint ramps in over one phase
+    duration; a joint stretched beyond factor * min dist breaks and ramps
+    out."""
+    system = walls_afm_system(small_open)
+    p = _single_term(1, "lj_wall")
+    p.dt = 0.005
+    p.lj_walls_phase_duration = 0.05  # 10 steps
+    ora = Oracle(p, system)
+    st, jo, rt = ora.get_wall_state(0)
+    assert np.all(st == 0)
+    ora.step(1)
+    st, jo, rt = ora.get_wall_state(0)
+    r = system.r0
+    x = r[0] - system.lj_walls[0, 0]
+    joined = np.abs(x) < p.lj_walls_min_dist
+    assert joined.sum() > 0 and np.array_equal(st == 1, joined)
+    assert np.allclose(jo[joined, 0], system.lj_walls[0, 0]) and np.allclose(jo[joined, 1:], r[1:, joined].T)
+    assert np.all(rt[joined] == 0.0)
+    # the force toward the joint ramps with t / phase duration
+    k = _abi.TERM["lj_wall"]
+    ora.step(4)
+    y, _, _, t = ora.get_state(0)
+    _, V = ora.eval_forces(1 << k)
+    i = np.flatnonzero(joined)
+    d = np.linalg.norm(jo[i] - y[0][:, i].T, axis=1)
+    rmin = p.lj_walls_min_dist
+    lj = np.clip((rmin / d) ** 12 - 2 * (rmin / d) ** 6, -1e3, 1e3)  # lj.h clamps V
+    assert 0 < t < 0.05
+    assert V[0, k] == pytest.approx((lj * min(t, 0.05) / 0.05).sum(), rel=1e-12)
+    # pull one joined bead far from its joint: breaks once saturated, then frees
+    ora.step(10)
+    y, _, _, t = ora.get_state(0)
+    b = i[0]
+    y[0][0, b] += 3 * rmin * p.lj_walls_breaking_factor
+    ora.set_state(0, y, ora.get_state(0)[2], t)
+    ora.step(1)
+    st2, _, rt2 = ora.get_wall_state(0)
+    assert st2[b] == 2 and rt2[b] == pytest.approx(t)
+    ora.step(12)
+    assert ora.get_wall_state(0)[0][b] in (0, 1)  # freed (and maybe joined again)
+
+
+def test_afm_forces(small_open):
+    system = walls_afm_system(small_open)
+    p = _single_term(1, "vel_afm")
+    p.force_afm_enabled = 1
+    ora = Oracle(p, system)
+    F, V = ora.eval_forces()
+    # force tip: the constant force; velocity tip at t = 0 sits on its bead
+    assert np.allclose(F[:, -1], [0.3, -0.2, 0.1], rtol=0, atol=0)
+    assert np.abs(F[:, 0]).max() == 0 or np.all(np.isnan(F[:, 0]))
+    y, _, rng, _ = ora.get_state(0)
+    ora.set_state(0, y, rng, 2.0)  # the tip has moved 0.1 along +z
+    F, V = ora.eval_forces()
+    H1 = p.vel_afm_H1
+    assert np.allclose(F[:, 0], [0.0, 0.0, H1 * 0.1], rtol=1e-12, atol=1e-15)
+    assert V[0, _abi.TERM["vel_afm"]] == 0 and V[0, _abi.TERM["force_afm"]] == 0

@@ -16,7 +16,7 @@ OK, ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_NONFINITE, ERR_STATE, ERR_NO_DEVICE = ra
 
 # xmd_term
 TERMS = ["tether", "chiral", "nat_ang", "cnd", "snd", "pauli", "go",
-         "const_es", "rel_es", "qa", "pid", "nat_ss"]
+         "const_es", "rel_es", "qa", "pid", "nat_ss", "solid_wall", "lj_wall", "vel_afm", "force_afm"]
 TERM = {name: k for k, name in enumerate(TERMS)}
 NUM_TERMS = len(TERMS)
 ALL_TERMS = (1 << NUM_TERMS) - 1
@@ -62,6 +62,11 @@ class XmdParams(C.Structure):
         ("aa_polarization", _i8 * NUM_AA),
         ("force_lanes", _i32),
         ("nat_ss_enabled", _i32), ("nat_ss_H1", _d), ("nat_ss_equil_dist", _d),
+        ("solid_walls_enabled", _i32), ("lj_walls_enabled", _i32),
+        ("solid_walls_depth", _d), ("solid_walls_cutoff", _d),
+        ("lj_walls_min_dist", _d), ("lj_walls_breaking_factor", _d), ("lj_walls_phase_duration", _d),
+        ("vel_afm_enabled", _i32), ("force_afm_enabled", _i32),
+        ("vel_afm_H1", _d), ("vel_afm_H2", _d),
     ]
 
 
@@ -75,6 +80,11 @@ class XmdTopology(C.Structure):
         ("tether_len", C.POINTER(_d)), ("nat_theta", C.POINTER(_d)), ("nat_phi", C.POINTER(_d)),
         ("nat_chir", C.POINTER(_d)), ("nat_chir_factor", C.POINTER(_d)),
         ("n_ssbonds", _i32), ("ss_i1", C.POINTER(_i32)), ("ss_i2", C.POINTER(_i32)),
+        ("n_solid_walls", _i32), ("solid_walls", C.POINTER(_d)),
+        ("n_lj_walls", _i32), ("lj_walls", C.POINTER(_d)),
+        ("n_vel_afm", _i32), ("vel_afm_idx", C.POINTER(_i32)), ("vel_afm_orig", C.POINTER(_d)),
+        ("vel_afm_v", C.POINTER(_d)),
+        ("n_force_afm", _i32), ("force_afm_idx", C.POINTER(_i32)), ("force_afm_F", C.POINTER(_d)),
     ]
 
 

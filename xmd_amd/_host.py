@@ -108,6 +108,24 @@ class HostSystem:
                                              contacts.ctypes.data_as(C.POINTER(_abi.XmdQaContact)),
                                              C.c_int64(len(contacts)), _abi.ptr(sync, C.c_int8)))
 
+    def get_wall_state(self, rep):
+        """LJ attractive wall state (status, joint, ref_t), per bead x wall."""
+        k = self._rn(rep) * len(self.system.lj_walls)
+        st = np.zeros(max(k, 1), dtype=np.int8)
+        jo = np.zeros(max(3 * k, 1))
+        rt = np.zeros(max(k, 1))
+        self._check(self._fn("get_wall_state")(self.handle, rep, _abi.ptr(st, C.c_int8), _abi.ptr(jo, C.c_double),
+                                               _abi.ptr(rt, C.c_double)))
+        return st[:k], jo[:3 * k].reshape(k, 3), rt[:k]
+
+    def set_wall_state(self, rep, status, joint, ref_t):
+        st = np.ascontiguousarray(status, dtype=np.int8)
+        jo = np.ascontiguousarray(joint, dtype=np.float64).reshape(-1)
+        rt = np.ascontiguousarray(ref_t, dtype=np.float64)
+        assert st.size == self._rn(rep) * len(self.system.lj_walls) and jo.size == 3 * st.size
+        self._check(self._fn("set_wall_state")(self.handle, rep, _abi.ptr(st, C.c_int8), _abi.ptr(jo, C.c_double),
+                                               _abi.ptr(rt, C.c_double)))
+
     def energy(self):
         V = np.zeros(self.nrep)
         self._check(self._fn("get_energy")(self.handle, _abi.ptr(V, C.c_double)))

@@ -12,14 +12,17 @@ using namespace xmd_dev;
 constexpr int WAVE = 64;
 constexpr int BLK = 256;
 constexpr int MAXQ = 16;
-constexpr int NUM_TERMS = 12;
+constexpr int NUM_TERMS = 16;
 
 // F_NATSS marks a disulfide pair of the list (native_ssbonds): F_NATIVE is
 // set too, so no non-native term takes it.
 enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16, F_NATSS = 32 };
 enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
 enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
-enum : int { T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID, T_NAT_SS };
+enum : int {
+    T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID, T_NAT_SS,
+    T_SOLID_WALL, T_LJ_WALL, T_VEL_AFM, T_FORCE_AFM
+};
 
 // Derived parameters in double (list and QA decisions, integrator).
 struct Derived {
@@ -33,6 +36,10 @@ struct Derived {
     // while within ss_cut + pad (FP64 in both forms)
     int ss_on;
     double ss_H1, ss_equil, ss_cut;
+    // walls (planes in Dev) and AFM tips, FP64 in both forms
+    int n_solid, n_ljw;
+    double solid_eps, solid_cut, ljw_min, ljw_factor, ljw_cyc, ljw_cyc_inv;
+    double vafm_H1, vafm_H2;
     QaParams qa;
 };
 
@@ -65,6 +72,14 @@ struct Dev {
     // disulfide bonds (rows of slot s: ss_start[s]..ss_start[s+1])
     const int *ss_start, *ss_j;
     uint8_t *ss_act;
+    // wall planes (6 doubles: origin, normal); LJ attractive wall state per
+    // slot x wall (0 FREE, 1 FORMING_OR_FORMED, 2 BREAKING), joint, time
+    const double *solid_w, *lj_w;
+    int8_t *ljw_status;
+    double *ljw_joint, *ljw_ref;
+    // AFM tips of slot s: vafm_start[s].. (origin, velocity), fafm_start[s].. (force)
+    const int *vafm_start, *fafm_start;
+    const double *vafm_orig, *vafm_v, *fafm_F;
     // neighbour rows
     int nl_w;  // row capacity per slot (rows are fixed-width sliced ELL)
     int *slice_start, *nl_len, *nl_j;
@@ -137,7 +152,9 @@ template <typename T> struct TermTab {
 // translation unit of its own compiled with multiply-add contraction).
 // Lanes per bead: 4, 8 (default) or 16, chosen by the autotuner.
 constexpr int FORCE_LPB = 8;
+// mutate: the launch is a step's force pass (LJ attractive wall pairs change
+// state); 0 for evaluations and timing launches.
 void launch_force(bool mixed, bool eval, int lpb, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
-                  const TermTab<float> &Tf, const TermTab<double> &Td, int p);
+                  const TermTab<float> &Tf, const TermTab<double> &Td, int p, int mutate);
 
 }  // namespace xmd_k

@@ -169,9 +169,9 @@ struct Part {
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));
         const bool mixed = P.precision == XMD_PREC_MIXED;
-        launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur);  // warm-up
+        launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur, 0);  // warm-up
         HIPCHK(hipEventRecord(a, stream));
-        for (int k = 0; k < reps; ++k) launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur);
+        for (int k = 0; k < reps; ++k) launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur, 0);
         HIPCHK(hipEventRecord(b, stream));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0;
@@ -312,7 +312,7 @@ struct Part {
             launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
         }
         launch("k_force", [&] {
-            launch_force(P.precision == XMD_PREC_MIXED, false, lpb, nslot, stream, D, G, TTf, TTd, p);
+            launch_force(P.precision == XMD_PREC_MIXED, false, lpb, nslot, stream, D, G, TTf, TTd, p, 1);
         });
         launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
         launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1); });
@@ -412,6 +412,15 @@ void derive(Part &E, double go_cut) {
     G.ss_equil = p.nat_ss_equil_dist;
     G.ss_cut = 2.0 * p.nat_ss_equil_dist;
     if (p.nat_ss_enabled) mc = std::max(mc, G.ss_cut);
+    // walls/solid.cpp, walls/lj_attr.cpp (main.cpp:836-842), velocity_afm.cpp
+    G.solid_eps = p.solid_walls_depth;
+    G.solid_cut = p.solid_walls_cutoff;
+    G.ljw_min = p.lj_walls_min_dist;
+    G.ljw_factor = p.lj_walls_breaking_factor * std::pow(2.0, -1.0 / 6.0);
+    G.ljw_cyc = p.lj_walls_phase_duration;
+    G.ljw_cyc_inv = 1.0 / p.lj_walls_phase_duration;
+    G.vafm_H1 = p.vel_afm_H1;
+    G.vafm_H2 = p.vel_afm_H2;
     if (p.max_cutoff_override > 0) mc = p.max_cutoff_override;
     G.pad = p.pad_factor * mc;
     G.req_r = mc + G.pad;
@@ -484,6 +493,10 @@ void derive(Part &E, double go_cut) {
     if (p.qa_enabled) m |= 1u << T_QA;
     if (p.pid_enabled) m |= 1u << T_PID;
     if (p.nat_ss_enabled) m |= 1u << T_NAT_SS;
+    if (p.solid_walls_enabled) m |= 1u << T_SOLID_WALL;
+    if (p.lj_walls_enabled) m |= 1u << T_LJ_WALL;
+    if (p.vel_afm_enabled) m |= 1u << T_VEL_AFM;
+    if (p.force_afm_enabled) m |= 1u << T_FORCE_AFM;
     E.mask = m;
     E.TTd.mask = m;
     E.TTf.mask = m;
@@ -636,6 +649,44 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
         ss_j.insert(ss_j.end(), srow[s].begin(), srow[s].end());
     }
     ss_start[n] = (int)ss_j.size();
+    // walls (the same planes in every replica) and AFM tips by slot
+    if (t->n_solid_walls < 0 || t->n_lj_walls < 0 || (t->n_solid_walls > 0 && !t->solid_walls) ||
+        (t->n_lj_walls > 0 && !t->lj_walls))
+        throw Fail{XMD_ERR_ARG, "bad wall arrays"};
+    std::vector<double> solid_w(t->solid_walls, t->solid_walls + 6 * (size_t)t->n_solid_walls);
+    std::vector<double> lj_w(t->lj_walls, t->lj_walls + 6 * (size_t)t->n_lj_walls);
+    E.G.n_solid = t->n_solid_walls;
+    E.G.n_ljw = t->n_lj_walls;
+    std::vector<std::vector<int>> vtip(n), ftip(n);
+    for (int c = 0; c < t->n_vel_afm; ++c) {
+        int g = t->vel_afm_idx[c];
+        if (g < 0 || g >= E.nbeads) throw Fail{XMD_ERR_ARG, "velocity AFM tip outside the beads"};
+        vtip[E.slot_of_bead[g]].push_back(c);
+    }
+    for (int c = 0; c < t->n_force_afm; ++c) {
+        int g = t->force_afm_idx[c];
+        if (g < 0 || g >= E.nbeads) throw Fail{XMD_ERR_ARG, "force AFM tip outside the beads"};
+        ftip[E.slot_of_bead[g]].push_back(c);
+    }
+    std::vector<int> vafm_start(n + 1, 0), fafm_start(n + 1, 0);
+    std::vector<double> vafm_orig, vafm_v, fafm_F;
+    for (int s = 0; s < n; ++s) {
+        vafm_start[s] = (int)vafm_v.size() / 3;
+        for (int c : vtip[s]) {
+            const double *o = t->vel_afm_orig + 3 * c, *v = t->vel_afm_v + 3 * c;
+            int g = t->vel_afm_idx[c];
+            for (int d = 0; d < 3; ++d) {
+                // no origin: the bead's initial position (D27)
+                vafm_orig.push_back(std::isnan(o[0]) ? t->r0[(size_t)d * E.nbeads + g] : o[d]);
+                vafm_v.push_back(v[d]);
+            }
+        }
+        fafm_start[s] = (int)fafm_F.size() / 3;
+        for (int c : ftip[s])
+            for (int d = 0; d < 3; ++d) fafm_F.push_back(t->force_afm_F[3 * c + d]);
+    }
+    vafm_start[n] = (int)vafm_v.size() / 3;
+    fafm_start[n] = (int)fafm_F.size() / 3;
     // bonded natives in slot space
     std::vector<double> teth(n, P.tether_def_length), th(n, NAN), ph(n, NAN), ch(n, NAN), cf(n, NAN);
     for (int g = 0; g < E.nbeads; ++g) {
@@ -761,6 +812,16 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.ss_start = E.upload(ss_start);
     D.ss_j = E.upload(ss_j);
     D.ss_act = E.alloc<uint8_t>(ss_j.size());
+    D.solid_w = E.upload(solid_w);
+    D.lj_w = E.upload(lj_w);
+    D.ljw_status = E.alloc<int8_t>((size_t)n * t->n_lj_walls);
+    D.ljw_joint = E.alloc<double>(3 * (size_t)n * t->n_lj_walls);
+    D.ljw_ref = E.alloc<double>((size_t)n * t->n_lj_walls);
+    D.vafm_start = E.upload(vafm_start);
+    D.fafm_start = E.upload(fafm_start);
+    D.vafm_orig = E.upload(vafm_orig);
+    D.vafm_v = E.upload(vafm_v);
+    D.fafm_F = E.upload(fafm_F);
     D.sync = E.upload(sync);
     D.qslot = E.alloc<int>((size_t)MAXQ * n);
     D.qn = E.alloc<int>(n);
@@ -1105,7 +1166,7 @@ static xmd_status part_eval_forces(Part *e, uint32_t term_mask, double *F, doubl
         TermTab<double> Td = e->TTd;
         Tf.mask = m;
         Td.mask = m;
-        launch_force(e->P.precision == XMD_PREC_MIXED, true, e->lpb, e->nslot, e->stream, e->D, e->G, Tf, Td, p);
+        launch_force(e->P.precision == XMD_PREC_MIXED, true, e->lpb, e->nslot, e->stream, e->D, e->G, Tf, Td, p, 0);
         k_gather_force<<<nb, BLK, 0, e->stream>>>(e->D);
         HIPCHK(hipGetLastError());
         int n = e->nslot;
@@ -1173,6 +1234,36 @@ static xmd_status part_get_qa_state(Part *e, int32_t rep, xmd_qa_contact *c, int
             auto sy = e->get(e->D.sync + s0, nb);
             std::memcpy(sync, sy.data(), 4 * (size_t)nb);
         }
+    });
+}
+
+static xmd_status part_get_wall_state(Part *e, int32_t rep, int8_t *status, double *joint, double *ref_t) {
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
+        const size_t w = (size_t)e->G.n_ljw, s0 = (size_t)e->rep_slot_off[rep], k = (size_t)e->rep_n[rep] * w;
+        if (!k) return;
+        auto st = e->get(e->D.ljw_status + s0 * w, k);
+        auto jo = e->get(e->D.ljw_joint + 3 * s0 * w, 3 * k);
+        auto rt = e->get(e->D.ljw_ref + s0 * w, k);
+        if (status) std::memcpy(status, st.data(), k);
+        if (joint) std::memcpy(joint, jo.data(), 3 * k * sizeof(double));
+        if (ref_t) std::memcpy(ref_t, rt.data(), k * sizeof(double));
+    });
+}
+
+static xmd_status part_set_wall_state(Part *e, int32_t rep, const int8_t *status, const double *joint,
+                                      const double *ref_t) {
+    return run_guarded(e->err, [&]() {
+        if (rep < 0 || rep >= e->nrep) throw Fail{XMD_ERR_ARG, "bad replica"};
+        const size_t w = (size_t)e->G.n_ljw, s0 = (size_t)e->rep_slot_off[rep], k = (size_t)e->rep_n[rep] * w;
+        if (!k) return;
+        if (!status || !joint || !ref_t) throw Fail{XMD_ERR_ARG, "null wall state"};
+        for (size_t q = 0; q < k; ++q)
+            if (status[q] < 0 || status[q] > 2) throw Fail{XMD_ERR_ARG, "bad wall status"};
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy(e->D.ljw_status + s0 * w, status, k, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->D.ljw_joint + 3 * s0 * w, joint, 3 * k * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->D.ljw_ref + s0 * w, ref_t, k * sizeof(double), hipMemcpyHostToDevice));
     });
 }
 
@@ -1356,7 +1447,8 @@ xmd_status on_part(xmd_engine *e, Part *q, xmd_status st) {
 // bead; `store` keeps the arrays it points into alive.
 struct SubTopo {
     xmd_topology t{};
-    std::vector<int32_t> rep_off, chain_off, i1, i2, ss1, ss2;
+    std::vector<int32_t> rep_off, chain_off, i1, i2, ss1, ss2, vidx, fidx;
+    std::vector<double> vorig, vvel, fF;
     std::vector<double> r0, dist;
 };
 void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
@@ -1404,6 +1496,28 @@ void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
     o.t.n_ssbonds = (int32_t)o.ss1.size();
     o.t.ss_i1 = o.ss1.data();
     o.t.ss_i2 = o.ss2.data();
+    o.t.n_solid_walls = t->n_solid_walls;
+    o.t.solid_walls = t->solid_walls;
+    o.t.n_lj_walls = t->n_lj_walls;
+    o.t.lj_walls = t->lj_walls;
+    for (int k = 0; k < t->n_vel_afm; ++k)
+        if (t->vel_afm_idx[k] >= b0 && t->vel_afm_idx[k] < b1) {
+            o.vidx.push_back(t->vel_afm_idx[k] - b0);
+            o.vorig.insert(o.vorig.end(), t->vel_afm_orig + 3 * k, t->vel_afm_orig + 3 * k + 3);
+            o.vvel.insert(o.vvel.end(), t->vel_afm_v + 3 * k, t->vel_afm_v + 3 * k + 3);
+        }
+    for (int k = 0; k < t->n_force_afm; ++k)
+        if (t->force_afm_idx[k] >= b0 && t->force_afm_idx[k] < b1) {
+            o.fidx.push_back(t->force_afm_idx[k] - b0);
+            o.fF.insert(o.fF.end(), t->force_afm_F + 3 * k, t->force_afm_F + 3 * k + 3);
+        }
+    o.t.n_vel_afm = (int32_t)o.vidx.size();
+    o.t.vel_afm_idx = o.vidx.data();
+    o.t.vel_afm_orig = o.vorig.data();
+    o.t.vel_afm_v = o.vvel.data();
+    o.t.n_force_afm = (int32_t)o.fidx.size();
+    o.t.force_afm_idx = o.fidx.data();
+    o.t.force_afm_F = o.fF.data();
 }
 
 void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
@@ -1511,6 +1625,13 @@ xmd_status xmd_get_qa_state(xmd_engine *e, int32_t rep, xmd_qa_contact *c, int64
 }
 xmd_status xmd_set_qa_state(xmd_engine *e, int32_t rep, const xmd_qa_contact *c, int64_t nc, const int8_t *sync) {
     XMD_REPLICA_CALL(part_set_qa_state(q, lr, c, nc, sync))
+}
+xmd_status xmd_get_wall_state(xmd_engine *e, int32_t rep, int8_t *status, double *joint, double *ref_t) {
+    XMD_REPLICA_CALL(part_get_wall_state(q, lr, status, joint, ref_t))
+}
+xmd_status xmd_set_wall_state(xmd_engine *e, int32_t rep, const int8_t *status, const double *joint,
+                              const double *ref_t) {
+    XMD_REPLICA_CALL(part_set_wall_state(q, lr, status, joint, ref_t))
 }
 
 xmd_status xmd_rebuild_lists(xmd_engine *e) {

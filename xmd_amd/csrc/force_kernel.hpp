@@ -42,7 +42,7 @@ template <int LPB> __device__ inline double lane_sum(double v) {
 
 template <typename T, bool EVAL, int LPB>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
-void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
+void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     const int n = D.nslot;
     const int t = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     const int s = t / LPB, sub = t % LPB;
@@ -199,6 +199,77 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p) {
                     if (owner) addV(T_PID, (double)v);
                 }
             }
+        }
+        // walls and AFM tips (walls/solid.cpp:19-30, walls/lj_attr.cpp:7-49,
+        // velocity_afm.cpp:12-24, force_afm.cpp:11-14; D25-D27), FP64 in
+        // both forms, on the bead's last lane
+        constexpr uint32_t WALL_AFM =
+            (1u << T_SOLID_WALL) | (1u << T_LJ_WALL) | (1u << T_VEL_AFM) | (1u << T_FORCE_AFM);
+        if ((mask & WALL_AFM) && sub == LPB - 1) {
+            const double tnow = D.tt[p][rep];
+            const v3<double> rw = wrap(box, ri);  // box.wrap of the position
+            if (mask & (1u << T_SOLID_WALL))
+                for (int w = 0; w < G.n_solid; ++w) {
+                    const double *pl = D.solid_w + 6 * w;
+                    const v3<double> nn = {pl[3], pl[4], pl[5]};
+                    const double x = dot(nn, rw - v3<double>{pl[0], pl[1], pl[2]});
+                    if (x < G.solid_cut) {
+                        const double x3 = x * x * x;
+                        const double Vw = G.solid_eps / (9.0 * (x3 * x3 * x3));
+                        add3(Fx, Fy, Fz, nn * (9.0 * Vw / x));
+                        addV(T_SOLID_WALL, Vw);
+                    }
+                }
+            if (mask & (1u << T_LJ_WALL))
+                for (int w = 0; w < G.n_ljw; ++w) {
+                    const size_t k = (size_t)s * G.n_ljw + w;
+                    const double *pl = D.lj_w + 6 * w;
+                    const v3<double> nn = {pl[3], pl[4], pl[5]};
+                    const int st = D.ljw_status[k];
+                    if (st == 0) {
+                        const double x = dot(nn, rw - v3<double>{pl[0], pl[1], pl[2]});
+                        if (fabs(x) < G.ljw_min && mutate) {
+                            D.ljw_status[k] = 1;
+                            const v3<double> j = rw - nn * x;
+                            D.ljw_joint[3 * k] = j.x, D.ljw_joint[3 * k + 1] = j.y, D.ljw_joint[3 * k + 2] = j.z;
+                            D.ljw_ref[k] = tnow;
+                        }
+                        continue;
+                    }
+                    double sat = fmin(tnow - D.ljw_ref[k], G.ljw_cyc) * G.ljw_cyc_inv;
+                    if (st == 2) sat = 1.0 - sat;
+                    const v3<double> j = {D.ljw_joint[3 * k], D.ljw_joint[3 * k + 1], D.ljw_joint[3 * k + 2]};
+                    const v3<double> r12 = j - rw;
+                    const double rn = 1.0 / norm(r12);
+                    const VdV<double> e = lj<double>(1.0, G.ljw_min, rn);
+                    addV(T_LJ_WALL, e.V * sat);
+                    add3(Fx, Fy, Fz, r12 * (sat * e.dV * rn));
+                    if (mutate) {
+                        if (st == 1 && sat == 1.0) {
+                            if (G.ljw_factor * G.ljw_min * rn < 1.0) {
+                                D.ljw_status[k] = 2;
+                                D.ljw_ref[k] = tnow;
+                            }
+                        } else if (st == 2 && sat == 0.0) {
+                            D.ljw_status[k] = 0;
+                        }
+                    }
+                }
+            if (mask & (1u << T_VEL_AFM))
+                for (int k = D.vafm_start[s]; k < D.vafm_start[s + 1]; ++k) {
+                    const v3<double> tip = v3<double>{D.vafm_orig[3 * k], D.vafm_orig[3 * k + 1], D.vafm_orig[3 * k + 2]} +
+                                           tnow * v3<double>{D.vafm_v[3 * k], D.vafm_v[3 * k + 1], D.vafm_v[3 * k + 2]};
+                    const v3<double> ra = tip - ri;
+                    const double rn = norm(ra);
+                    const VdV<double> e = harmonic<double>(G.vafm_H1, G.vafm_H2, 0.0, rn);
+                    add3(Fx, Fy, Fz, ra * (e.dV / rn));
+                }
+            if (mask & (1u << T_FORCE_AFM))
+                for (int k = D.fafm_start[s]; k < D.fafm_start[s + 1]; ++k) {
+                    Fx += D.fafm_F[3 * k];
+                    Fy += D.fafm_F[3 * k + 1];
+                    Fz += D.fafm_F[3 * k + 2];
+                }
         }
         // quasi-adiabatic contacts through the bead's contact slots
         if (mask & (1u << T_QA)) {

@@ -6,22 +6,22 @@ namespace xmd_k {
 
 template <int LPB>
 static void launch_lpb(bool mixed, bool eval, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
-                       const TermTab<float> &Tf, const TermTab<double> &Td, int p) {
+                       const TermTab<float> &Tf, const TermTab<double> &Td, int p, int mutate) {
     const int grid = nslot * LPB / BLK;
     if (mixed) {
-        if (eval) k_force<float, true, LPB><<<grid, BLK, 0, stream>>>(D, G, Tf, p);
-        else k_force<float, false, LPB><<<grid, BLK, 0, stream>>>(D, G, Tf, p);
+        if (eval) k_force<float, true, LPB><<<grid, BLK, 0, stream>>>(D, G, Tf, p, mutate);
+        else k_force<float, false, LPB><<<grid, BLK, 0, stream>>>(D, G, Tf, p, mutate);
     } else {
-        if (eval) k_force<double, true, LPB><<<grid, BLK, 0, stream>>>(D, G, Td, p);
-        else k_force<double, false, LPB><<<grid, BLK, 0, stream>>>(D, G, Td, p);
+        if (eval) k_force<double, true, LPB><<<grid, BLK, 0, stream>>>(D, G, Td, p, mutate);
+        else k_force<double, false, LPB><<<grid, BLK, 0, stream>>>(D, G, Td, p, mutate);
     }
 }
 
 void launch_force(bool mixed, bool eval, int lpb, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
-                  const TermTab<float> &Tf, const TermTab<double> &Td, int p) {
-    if (lpb == 4) launch_lpb<4>(mixed, eval, nslot, stream, D, G, Tf, Td, p);
-    else if (lpb == 16) launch_lpb<16>(mixed, eval, nslot, stream, D, G, Tf, Td, p);
-    else launch_lpb<8>(mixed, eval, nslot, stream, D, G, Tf, Td, p);
+                  const TermTab<float> &Tf, const TermTab<double> &Td, int p, int mutate) {
+    if (lpb == 4) launch_lpb<4>(mixed, eval, nslot, stream, D, G, Tf, Td, p, mutate);
+    else if (lpb == 16) launch_lpb<16>(mixed, eval, nslot, stream, D, G, Tf, Td, p, mutate);
+    else launch_lpb<8>(mixed, eval, nslot, stream, D, G, Tf, Td, p, mutate);
 }
 
 }  // namespace xmd_k

@@ -512,4 +512,15 @@ def from_input(path_or_dict=None, base_dir=None, replicas=1, **kw):
         systems.append(to_system(m, T, state))
         models.append(m)
     tree["general"]["seed"] = seed
-    return params, concat(systems), models
+    system = concat(systems)
+    wt = P.walls_and_tips(tree)
+    n = [s.n_beads for s in systems]
+    off = np.cumsum([0] + n[:-1])
+    for key in ("vel_afm", "force_afm"):
+        idx = wt.pop(f"{key}_idx")
+        if np.any((idx < 0) | (idx >= min(n))):
+            raise ValueError(f"{key} tip pulls a residue outside the model")
+        setattr(system, f"{key}_idx", np.concatenate([idx + o for o in off]).astype(np.int32))
+    for key, val in wt.items():
+        setattr(system, key, val if key.endswith("walls") else np.concatenate([val] * len(systems)))
+    return params, system, models

@@ -18,6 +18,7 @@ import math
 import os
 import re
 
+import numpy as np
 import yaml
 
 from . import _abi
@@ -247,6 +248,17 @@ def to_xmd_params(tree, base_dir=".", precision=_abi.PREC_FP64, rng_exact=True,
     ss = tree["native ssbonds"]
     p.nat_ss_enabled, p.nat_ss_H1 = int(_bool(ss["enabled"])), q(ss["H1"])
     p.nat_ss_equil_dist = q(ss["equilibrium dist"])
+    # params.cpp:223-270 (planes and tips go to the topology, walls_and_tips)
+    w = tree["solid walls"]
+    p.solid_walls_enabled = int(_bool(w["enabled"]))
+    p.solid_walls_depth, p.solid_walls_cutoff = q(w["depth"]), q(w["cutoff"])
+    w = tree["LJ attractive walls"]
+    p.lj_walls_enabled = int(_bool(w["enabled"]))
+    p.lj_walls_min_dist, p.lj_walls_breaking_factor = q(w["wall min dist"]), float(w["breaking factor"])
+    p.lj_walls_phase_duration = q(w["phase duration"])
+    w = tree["velocity AFM"]
+    p.vel_afm_enabled, p.vel_afm_H1, p.vel_afm_H2 = int(_bool(w["enabled"])), q(w["H1"]), q(w["H2"])
+    p.force_afm_enabled = int(_bool(tree["force AFM"]["enabled"]))
     e = tree["const ES"]
     p.const_es_enabled = int(_bool(e["enabled"]))
     p.const_es_screen, p.const_es_permittivity = q(e["screening distance"]), q(e["permittivity"])
@@ -307,6 +319,31 @@ def to_xmd_params(tree, base_dir=".", precision=_abi.PREC_FP64, rng_exact=True,
     p.qa_min_cos_hh = float(qa["min |cos(h, h)| for bb"])
     p.qa_max_cos_nr = float(qa["max cos(n, r)"])
     return p
+
+
+def _vec3(node):
+    """A YAML [x, y, z] of quantities (xmd/src/types/vec3.cpp:10-19)."""
+    return [quantity(v) for v in node]
+
+
+def walls_and_tips(tree):
+    """Wall planes and AFM tips of a parameter tree, in the topology's
+    layout: planes (k, 6) = origin, normal; velocity tips (residue index,
+    origin or NaN, velocity); force tips (residue index, force)."""
+    def planes(sec):
+        return np.array([_vec3(pl["origin"]) + _vec3(pl["normal"]) for pl in (sec.get("planes") or [])],
+                        dtype=np.float64).reshape(-1, 6)
+    vt = tree["velocity AFM"].get("AFM tips") or []
+    ft = tree["force AFM"].get("AFM tips") or []
+    return dict(
+        solid_walls=planes(tree["solid walls"]), lj_walls=planes(tree["LJ attractive walls"]),
+        vel_afm_idx=np.array([int(t["pulled idx"]) for t in vt], dtype=np.int32),
+        vel_afm_orig=np.array([_vec3(t["origin"]) if t.get("origin") is not None else [np.nan] * 3 for t in vt],
+                              dtype=np.float64).reshape(-1, 3),
+        vel_afm_v=np.array([_vec3(t["v"]) for t in vt], dtype=np.float64).reshape(-1, 3),
+        force_afm_idx=np.array([int(t["pulled idx"]) for t in ft], dtype=np.int32),
+        force_afm_F=np.array([_vec3(t["F"]) for t in ft], dtype=np.float64).reshape(-1, 3),
+    )
 
 
 def from_yaml(path_or_dict=None, **kw):

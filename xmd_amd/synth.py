@@ -121,6 +121,16 @@ class System:
     nat_chir_factor: np.ndarray
     ss_i1: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     ss_i2: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    # wall planes (k, 6): origin, unit normal; the same for every replica
+    solid_walls: np.ndarray = field(default_factory=lambda: np.zeros((0, 6)))
+    lj_walls: np.ndarray = field(default_factory=lambda: np.zeros((0, 6)))
+    # AFM tips: global bead index, origin (NaN = initial position) and
+    # velocity, or constant force
+    vel_afm_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    vel_afm_orig: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    vel_afm_v: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    force_afm_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    force_afm_F: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -154,6 +164,13 @@ class System:
             nat_chir_factor=np.ascontiguousarray(self.nat_chir_factor, dtype=np.float64),
             ss_i1=np.ascontiguousarray(self.ss_i1, dtype=np.int32),
             ss_i2=np.ascontiguousarray(self.ss_i2, dtype=np.int32),
+            solid_walls=np.ascontiguousarray(self.solid_walls, dtype=np.float64).reshape(-1),
+            lj_walls=np.ascontiguousarray(self.lj_walls, dtype=np.float64).reshape(-1),
+            vel_afm_idx=np.ascontiguousarray(self.vel_afm_idx, dtype=np.int32),
+            vel_afm_orig=np.ascontiguousarray(self.vel_afm_orig, dtype=np.float64).reshape(-1),
+            vel_afm_v=np.ascontiguousarray(self.vel_afm_v, dtype=np.float64).reshape(-1),
+            force_afm_idx=np.ascontiguousarray(self.force_afm_idx, dtype=np.int32),
+            force_afm_F=np.ascontiguousarray(self.force_afm_F, dtype=np.float64).reshape(-1),
         )
         t = _abi.XmdTopology()
         t.n_replicas = self.n_replicas
@@ -161,8 +178,13 @@ class System:
         t.n_chains = len(self.chain_offset) - 1
         t.n_native = len(self.nat_i1)
         t.n_ssbonds = len(self.ss_i1)
+        t.n_solid_walls = len(self.solid_walls)
+        t.n_lj_walls = len(self.lj_walls)
+        t.n_vel_afm = len(self.vel_afm_idx)
+        t.n_force_afm = len(self.force_afm_idx)
         types = dict(replica_offset=C.c_int32, chain_offset=C.c_int32, atype=C.c_int8,
-                     rng_state=C.c_uint64, nat_i1=C.c_int32, nat_i2=C.c_int32, ss_i1=C.c_int32, ss_i2=C.c_int32)
+                     rng_state=C.c_uint64, nat_i1=C.c_int32, nat_i2=C.c_int32, ss_i1=C.c_int32, ss_i2=C.c_int32,
+                     vel_afm_idx=C.c_int32, force_afm_idx=C.c_int32)
         for name, a in arrs.items():
             setattr(t, name, _abi.ptr(a, types.get(name, C.c_double)))
         t._arrays = arrs  # keep alive with the struct
@@ -259,7 +281,9 @@ C3_TERMS = dict(C0_TERMS, const_es=True)
 
 _FLAG = dict(go="go_enabled", pauli="pauli_enabled", tether="tether_enabled", chir="chir_enabled",
              nat_ang="nat_ang_enabled", cnd="cnd_enabled", snd="snd_enabled", const_es="const_es_enabled",
-             rel_es="rel_es_enabled", qa="qa_enabled", pid="pid_enabled", nat_ss="nat_ss_enabled")
+             rel_es="rel_es_enabled", qa="qa_enabled", pid="pid_enabled", nat_ss="nat_ss_enabled",
+             solid_wall="solid_walls_enabled", lj_wall="lj_walls_enabled", vel_afm="vel_afm_enabled",
+             force_afm="force_afm_enabled")
 
 
 def apply_terms(p, terms):
@@ -362,7 +386,13 @@ def replicate(system, temperatures, seeds):
         nat_i1=shifted(system.nat_i1), nat_i2=shifted(system.nat_i2), nat_dist=tile(system.nat_dist),
         tether_len=tile(system.tether_len), nat_theta=tile(system.nat_theta), nat_phi=tile(system.nat_phi),
         nat_chir=tile(system.nat_chir), nat_chir_factor=tile(system.nat_chir_factor),
-        ss_i1=shifted(system.ss_i1).astype(np.int32), ss_i2=shifted(system.ss_i2).astype(np.int32))
+        ss_i1=shifted(system.ss_i1).astype(np.int32), ss_i2=shifted(system.ss_i2).astype(np.int32),
+        solid_walls=system.solid_walls, lj_walls=system.lj_walls,
+        vel_afm_idx=shifted(system.vel_afm_idx).astype(np.int32),
+        vel_afm_orig=np.concatenate([system.vel_afm_orig.reshape(-1, 3)] * k),
+        vel_afm_v=np.concatenate([system.vel_afm_v.reshape(-1, 3)] * k),
+        force_afm_idx=shifted(system.force_afm_idx).astype(np.int32),
+        force_afm_F=np.concatenate([system.force_afm_F.reshape(-1, 3)] * k))
 
 
 def concat(systems):
@@ -381,7 +411,13 @@ def concat(systems):
         nat_i1=shift("nat_i1").astype(np.int32), nat_i2=shift("nat_i2").astype(np.int32), nat_dist=cat("nat_dist"),
         tether_len=cat("tether_len"), nat_theta=cat("nat_theta"), nat_phi=cat("nat_phi"),
         nat_chir=cat("nat_chir"), nat_chir_factor=cat("nat_chir_factor"),
-        ss_i1=shift("ss_i1").astype(np.int32), ss_i2=shift("ss_i2").astype(np.int32))
+        ss_i1=shift("ss_i1").astype(np.int32), ss_i2=shift("ss_i2").astype(np.int32),
+        solid_walls=systems[0].solid_walls, lj_walls=systems[0].lj_walls,
+        vel_afm_idx=shift("vel_afm_idx").astype(np.int32),
+        vel_afm_orig=np.concatenate([s.vel_afm_orig.reshape(-1, 3) for s in systems]),
+        vel_afm_v=np.concatenate([s.vel_afm_v.reshape(-1, 3) for s in systems]),
+        force_afm_idx=shift("force_afm_idx").astype(np.int32),
+        force_afm_F=np.concatenate([s.force_afm_F.reshape(-1, 3) for s in systems]))
 
 
 def perturbed_positions(system, seed, sigma_A=0.3):
