@@ -61,7 +61,9 @@
 //   D27 velocity AFM: the harmonic's rest length is 0 (velocity_afm.h:21
 //       default-constructs it; main.cpp sets H1, H2 only); a tip without
 //       an origin starts at its bead's initial position (main.cpp:884-886
-//       reads an undefined pulled_idx).  Neither AFM term adds energy
+//       reads an undefined pulled_idx), so no force while the tip sits
+//       on its bead (the limit of the harmonic's gradient; the literal
+//       expression is 0 * inf).  Neither AFM term adds energy
 //       (velocity_afm.cpp:21, force_afm.cpp:13).
 //   D24 disulfide pairs of the list (native_ssbonds) belong to neither
 //       native_contacts nor non_native, so no pair term but the bond sees
@@ -622,7 +624,9 @@ template <typename R> void eval_vel_afm(const Params<R> &P, Replica<R> &s, std::
         auto const &tip = s.vtips[k];
         V3<R> cur = tip.orig + s.t * tip.v;
         V3<R> r_afm = cur - s.r[tip.i];
-        R r_afm_n = norm(r_afm), r_afm_rn = R(1.0f) / r_afm_n;
+        R r_afm_n = norm(r_afm);
+        if (r_afm_n == R(0.0)) continue;  // D27: the force's limit, 0
+        R r_afm_rn = R(1.0f) / r_afm_n;
         V3<R> r_afm_u = r_afm * r_afm_rn;
         auto [V_, dV_dr] = Harmonic<R>{P.vafm_H1, P.vafm_H2, R(0.0)}(r_afm_n);
         (void)V_;

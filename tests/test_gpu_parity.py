@@ -451,20 +451,26 @@ def test_walls_and_afm(prec, tmp_path):
     if prec == _abi.PREC_MIXED:
         eng.close()
         return
-    eng.step(40)
-    ora.step(40)
-    y, _, rng, t = eng.get_state(0)
-    yo, _, rngo, to = ora.get_state(0)
-    assert rng == rngo and t == to
-    assert np.abs(y - yo).max() < 1e-9
-    st, jo, rt = eng.get_wall_state(0)
-    sto, joo, rto = ora.get_wall_state(0)
-    assert np.array_equal(st, sto) and np.array_equal(rt, rto) and np.abs(jo - joo).max() < 1e-9
-    assert (st == 1).sum() > 0
-    path = str(tmp_path / "ckpt.npz")
-    checkpoint.save(eng, path)
+    # pairs join at once, saturate after 10 steps, break and free later:
+    # at step 20 some are formed and some breaking, at step 40 all free
+    states = []
+    for _ in range(2):
+        eng.step(20)
+        ora.step(20)
+        y, _, rng, t = eng.get_state(0)
+        yo, _, rngo, to = ora.get_state(0)
+        assert rng == rngo and t == to
+        assert np.abs(y - yo).max() < 1e-9
+        st, jo, rt = eng.get_wall_state(0)
+        sto, joo, rto = ora.get_wall_state(0)
+        assert np.array_equal(st, sto) and np.array_equal(rt, rto) and np.abs(jo - joo).max() < 1e-9
+        states.append(st.copy())
+        if len(states) == 1:
+            ckpt = str(tmp_path / "ckpt.npz")
+            checkpoint.save(eng, ckpt)
+    assert {1, 2} <= set(states[0].tolist())
     eng2 = Engine(params, system)
-    checkpoint.load(eng2, path)
-    assert np.array_equal(eng2.get_wall_state(0)[0], st)
+    checkpoint.load(eng2, ckpt)
+    assert np.array_equal(eng2.get_wall_state(0)[0], states[0])
     eng.close()
     eng2.close()

@@ -1,7 +1,7 @@
 // force_kernel.hpp -- the per-bead force kernel (included by forces.hip
-// only, which is compiled with -ffp-contract=fast: the force arithmetic may
-// fuse multiply-adds; list, QA and noise code elsewhere keeps exact
-// evaluation order).
+// only, which is compiled with -ffp-contract=fast-honor-pragmas: the force
+// arithmetic may fuse multiply-adds, except in blocks that turn contraction
+// off; list, QA and noise code elsewhere keeps exact evaluation order).
 #pragma once
 #include "common.hpp"
 
@@ -206,6 +206,10 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
         constexpr uint32_t WALL_AFM =
             (1u << T_SOLID_WALL) | (1u << T_LJ_WALL) | (1u << T_VEL_AFM) | (1u << T_FORCE_AFM);
         if ((mask & WALL_AFM) && sub == LPB - 1) {
+            // ISO evaluation here: the pair states test saturation == 0 and
+            // == 1 exactly (lj_attr.cpp:36-43), and a fused 1 - m c^-1
+            // misses both
+#pragma clang fp contract(off)
             const double tnow = D.tt[p][rep];
             const v3<double> rw = wrap(box, ri);  // box.wrap of the position
             if (mask & (1u << T_SOLID_WALL))
@@ -261,6 +265,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                                            tnow * v3<double>{D.vafm_v[3 * k], D.vafm_v[3 * k + 1], D.vafm_v[3 * k + 2]};
                     const v3<double> ra = tip - ri;
                     const double rn = norm(ra);
+                    if (rn == 0.0) continue;  // the tip on its bead: no force (D27)
                     const VdV<double> e = harmonic<double>(G.vafm_H1, G.vafm_H2, 0.0, rn);
                     add3(Fx, Fy, Fz, ra * (e.dV / rn));
                 }

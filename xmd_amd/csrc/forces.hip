@@ -1,5 +1,5 @@
 // forces.hip -- translation unit of the force kernel, built with
-// -ffp-contract=fast (see force_kernel.hpp).
+// -ffp-contract=fast-honor-pragmas (see force_kernel.hpp).
 #include "force_kernel.hpp"
 
 namespace xmd_k {
