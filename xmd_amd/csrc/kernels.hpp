@@ -94,15 +94,14 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
             if (s_pre[mid] <= wi) lo = mid;
             else hi = mid - 1;
         }
-        const int blk = lo < D.nrep ? D.fp_base[lo] / BLK + (wi - s_pre[lo])
-                                    : nfp_chunks + D.ct_base[lo - D.nrep] / BLK + (wi - s_pre[lo]);
-        bool sift = blk < nfp_chunks;
+        // the segment names the replica and the chunk's place in it, so
+        // its base, live count and box load together
+        const bool sift = lo < D.nrep;
         if (sift && !mutate) continue;
-        int chunk = sift ? blk : blk - nfp_chunks;
-        int rep = sift ? D.fp_chunk_rep[chunk] : D.ct_chunk_rep[chunk];
-        if (rep < 0) continue;
-        int first = chunk * BLK - (sift ? D.fp_base[rep] : D.ct_base[rep]);
-        int count = sift ? D.fp_count[rep] : D.ct_count[rep];
+        const int rep = sift ? lo : lo - D.nrep;
+        const int first = (wi - s_pre[lo]) * BLK;
+        const int count = sift ? D.fp_count[rep] : D.ct_count[rep];
+        const int chunk = (sift ? D.fp_base[rep] : D.ct_base[rep]) / BLK + (wi - s_pre[lo]);
         if (first >= count) continue;
         int idx = chunk * BLK + tid;
         int local = first + tid;
@@ -391,6 +390,18 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
 __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p, int s, int rep, bool active, double Fx,
                                            double Fy, double Fz) {
     const int n = D.nslot;
+    double *y = D.y;
+    // the bead's state first: these loads overlap the noise's dependent
+    // chain (generator state -> jump tables -> correction table)
+    v3<double> y0{0, 0, 0}, y1{0, 0, 0}, y2{0, 0, 0}, y3{0, 0, 0}, y4{0, 0, 0}, y5{0, 0, 0}, r_orig{0, 0, 0};
+    if (active) {
+        y0 = ldp(D.pos[p], s);
+        y1 = ld3(y, n, s), y2 = ld3(y + 3 * n, n, s), y3 = ld3(y + 6 * n, n, s);
+        y4 = ld3(y + 9 * n, n, s), y5 = ld3(y + 12 * n, n, s);
+        r_orig = ldp(D.orig_r, s);
+    }
+    const double nsd = D.noise_sd[rep];
+    const Box box = D.box[rep];
     int local = s - D.rep_slot_off[rep];
     uint64_t S = wave_lane_state(D, D.rng[p][rep], local >> 6);
     float nz[3];
@@ -399,12 +410,7 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
     if (active && (!((rng_out(S) >> 32)) || !((rng_out(rng_step(rng_step(S))) >> 32)))) atomicOr(D.err, E_ZERO_DRAW);
     double disp = 0.0;
     if (active) {
-        double *y = D.y;
-        v3<double> y0 = ldp(D.pos[p], s);
-        v3<double> y1 = ld3(y, n, s), y2 = ld3(y + 3 * n, n, s), y3 = ld3(y + 6 * n, n, s);
-        v3<double> y4 = ld3(y + 9 * n, n, s), y5 = ld3(y + 12 * n, n, s);
         if (G.lang_on) {
-            double nsd = D.noise_sd[rep];
             v3<double> v = y1 * G.dt_inv;
             v3<double> a;
             a.x = Fx * 1.0 - G.gamma * v.x + nsd * (double)nz[0];
@@ -429,7 +435,7 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
         st3(y + 6 * n, n, s, y3);
         st3(y + 9 * n, n, s, y4);
         st3(y + 12 * n, n, s, y5);
-        v3<double> dr = r_uv(D.box[rep], y0, ldp(D.orig_r, s));
+        v3<double> dr = r_uv(box, y0, r_orig);
         disp = norm(dr);
         if (!(disp == disp) || !(fabs(y0.x) < 1e300)) disp = __builtin_nan("");
     }
@@ -450,23 +456,20 @@ __global__ __launch_bounds__(BLK) void k_integrate(Dev D, Derived G, int p) {
     int rep = D.slot_rep[s];
     uint8_t bf = D.bflags[s];
     bool active = bf & B_ACTIVE;
-    double Fx = 0, Fy = 0, Fz = 0;
-    if (active) {
-        Fx = D.Fbuf[s];
-        Fy = D.Fbuf[n + s];
-        Fz = D.Fbuf[2 * n + s];
-        const double *ps = D.pid_side;
-        if (bf & B_PREV) {
-            Fx += ps[3 * n + s - 1];
-            Fy += ps[4 * n + s - 1];
-            Fz += ps[5 * n + s - 1];
-        }
-        if (bf & B_NEXT) {
-            Fx += ps[s + 1];
-            Fy += ps[n + s + 1];
-            Fz += ps[2 * n + s + 1];
-        }
-    }
+    // branch-free loads (in-bounds slots stand in for missing neighbours) so
+    // that nothing waits for them before the state and noise loads issue;
+    // the selects keep the sum order F + prev + next
+    const bool hp = bf & B_PREV, hn = bf & B_NEXT;
+    const int sp = hp ? s - 1 : s, sn = hn ? s + 1 : s;
+    const double *ps = D.pid_side;
+    const double fx = D.Fbuf[s], fy = D.Fbuf[n + s], fz = D.Fbuf[2 * n + s];
+    const double px = ps[3 * n + sp], py = ps[4 * n + sp], pz = ps[5 * n + sp];
+    const double qx = ps[sn], qy = ps[n + sn], qz = ps[2 * n + sn];
+    double Fx = hp ? fx + px : fx, Fy = hp ? fy + py : fy, Fz = hp ? fz + pz : fz;
+    Fx = hn ? Fx + qx : Fx;
+    Fy = hn ? Fy + qy : Fy;
+    Fz = hn ? Fz + qz : Fz;
+    if (!active) Fx = Fy = Fz = 0.0;
     integrate_bead_impl(D, G, p, s, rep, active, Fx, Fy, Fz);
 }
 
