@@ -931,8 +931,43 @@ template <typename R> void update_lists(const Params<R> &P, Replica<R> &s) {
         }
     }
 }
-template <typename R> void rebuild(const Params<R> &P, Replica<R> &s) {
-    legacy_update(P, s);
+// The same list as legacy_update, built by OpenMP threads over i1 (each
+// row collected on its own, rows joined in i1 order, membership in the
+// native lists by binary search): for the test entry point that rebuilds
+// on request.  Stepping keeps legacy_update, serial as in the reference
+// (main.cpp:1052-1075 runs it on the master thread).
+template <typename R> void legacy_update_par(const Params<R> &P, Replica<R> &s) {
+    R pad = P.pad_factor * P.max_cutoff;
+    R req_r = P.max_cutoff + pad;
+    std::vector<std::vector<NlPair<R>>> rows(s.n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i1 = 0; i1 < s.n; ++i1) {
+        V3<R> r1 = s.r[i1];
+        int chain1 = s.chain[i1], seq1 = s.seq[i1];
+        for (int i2 = i1 + 1; i2 < s.n; ++i2) {
+            int chain2 = s.chain[i2], seq2 = s.seq[i2];
+            int diff = seq1 > seq2 ? seq1 - seq2 : seq2 - seq1;
+            if (chain1 == chain2 && diff < 3) continue;
+            R orig_dist = norm(s.box.r_uv(r1, s.r[i2]));
+            if (orig_dist < req_r) {
+                std::pair<int, int> p{i1, i2};
+                bool native = std::binary_search(s.all_nat_nl.begin(), s.all_nat_nl.end(), p);
+                bool ss = std::binary_search(s.all_ss.begin(), s.all_ss.end(), p);
+                rows[i1].push_back({i1, i2, orig_dist, native || ss, ss});
+            }
+        }
+    }
+    s.nl.clear();
+    for (auto &row : rows) s.nl.insert(s.nl.end(), row.begin(), row.end());
+    s.orig_pad = pad;
+    s.orig_r = s.r;
+    s.orig_box = s.box;
+    s.ref_t = s.t;
+    s.invalid = false;
+}
+template <typename R> void rebuild(const Params<R> &P, Replica<R> &s, bool par = false) {
+    if (par) legacy_update_par(P, s);
+    else legacy_update(P, s);
     update_lists(P, s);
     s.rebuilds++;
 }
@@ -1502,7 +1537,7 @@ int oracle_rebuild_lists(oracle_sys *h) {
     return dispatch(h, [&](auto &S) {
         for (auto &s : S.reps) {
             s.first_time = false;
-            rebuild(S.P, s);
+            rebuild(S.P, s, s.n >= 4096);
         }
         return 0;
     });

@@ -526,3 +526,18 @@ def test_afm_forces(small_open):
     H1 = p.vel_afm_H1
     assert np.allclose(F[:, 0], [0.0, 0.0, H1 * 0.1], rtol=1e-12, atol=1e-15)
     assert V[0, _abi.TERM["vel_afm"]] == 0 and V[0, _abi.TERM["force_afm"]] == 0
+
+
+def test_parallel_list_build_equals_serial():
+    """The oracle's threaded list build (its rebuild entry point, from 4096
+    beads) gives the serial legacy_update's lists in the same order."""
+    system = synth.config_system(2, chains=20, chain_len=250)
+    p = synth.config_params(2)
+    a, b = Oracle(p, system), Oracle(p, system)
+    a.rebuild_lists()  # threaded
+    b.eval_forces()    # first evaluation: serial build
+    for name in _abi.LISTS:
+        x1, x2 = a.get_list(0, name)
+        y1, y2 = b.get_list(0, name)
+        assert np.array_equal(x1, y1) and np.array_equal(x2, y2), name
+    assert len(a.get_list(0, "nl_all")[0]) > 10000
