@@ -76,6 +76,10 @@ def test_c3_all_replicas():
     params = synth.config_params(3)
     print("C3", system.n_replicas, system.n_beads, flush=True)
     eng, ora = Engine(params, system), Oracle(params, system)
+    for k in range(system.n_replicas):  # off the native geometry (tethers at rest)
+        r = _displaced(system, 1000 + k, rep=k)
+        eng.set_positions(k, r)
+        ora.set_positions(k, r)
     eng.rebuild_lists()
     ora.rebuild_lists()
     for k in range(system.n_replicas):
