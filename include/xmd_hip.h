@@ -66,7 +66,9 @@ typedef enum xmd_term {
     XMD_TERM_LJ_WALL = 13,  /* "LJ attractive walls"  src/forces/walls/lj_attr.cpp:7-49      */
     XMD_TERM_VEL_AFM = 14,  /* "velocity AFM"         src/forces/velocity_afm.cpp:12-24      */
     XMD_TERM_FORCE_AFM = 15,/* "force AFM"            src/forces/force_afm.cpp:11-14         */
-    XMD_NUM_TERMS = 16
+    XMD_TERM_HEUR_ANG = 16, /* "heurestic angles"     src/forces/angle/heurestic.cpp:29-59   */
+    XMD_TERM_HEUR_DIH = 17, /* "heurestic dihedrals"  src/forces/dihedral/heurestic.cpp:29-70 */
+    XMD_NUM_TERMS = 18
 } xmd_term;
 
 /* Lists produced by a Verlet rebuild (xmd/include/xmd/nl/data.h:8-14 and the
@@ -171,6 +173,14 @@ typedef struct xmd_params {
     /* "velocity AFM", "force AFM" (params.cpp:248-270; tips in xmd_topology) */
     int32_t vel_afm_enabled, force_afm_enabled;
     double vel_afm_H1, vel_afm_H2;
+    /* "heurestic angles", "heurestic dihedrals" (params.cpp:69-124; the
+       terms' tables, angle/heurestic.h, dihedral/heurestic.h), indexed by
+       the pair type 3*c1 + c2 with c = 0 GLY, 1 PRO, 2 any other residue:
+       angle V = sum_d a_d theta^d (d = 0..6, eps/rad^d); dihedral
+       V = c0 + c1 sin + c2 cos + c3 sin^2 + c4 cos^2 + c5 sin cos (eps) */
+    int32_t heur_ang_enabled, heur_dih_enabled;
+    double heur_ang_coeffs[9][7];
+    double heur_dih_coeffs[9][6];
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and
@@ -218,6 +228,11 @@ typedef struct xmd_topology {
     int32_t n_force_afm;
     const int32_t *force_afm_idx;
     const double *force_afm_F;    /* [3*n_force_afm] */
+    /* [n_beads] or NULL: bit 0 = the model has an angle i,i+1,i+2 without a
+       native value, bit 1 = a dihedral i..i+3 without one (the heuristic
+       terms' lists); the beads must be consecutive in one chain.  Typed by
+       the residues of beads i, i+1 (angle) and i+1, i+2 (dihedral). */
+    const uint8_t *heur_bonded;
 } xmd_topology;
 
 /* QA contact record (xmd/include/xmd/forces/qa/contact.h:15-16). */

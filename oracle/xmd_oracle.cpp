@@ -68,6 +68,19 @@
 //   D24 disulfide pairs of the list (native_ssbonds) belong to neither
 //       native_contacts nor non_native, so no pair term but the bond sees
 //       them (Debye-Hueckel reads native_contacts + non_native, D2).
+//   D28 heuristic angles and dihedrals: main.cpp never creates their
+//       kernels, so their lists are defined here as the model's angles and
+//       dihedrals without a native value (the topology's heur_bonded
+//       bits), typed by heur_ang_type(a1, a2) of the angle's beads 1, 2
+//       and heur_dih_type(a2, a3) of the dihedral's beads 2, 3
+//       (angle/heurestic.h, dihedral/heurestic.h argument names).
+//   D29 coefficient tables are read by position: 7 angle coefficients
+//       a0..a6 (POLY_DEG = 6; defaults.yml rows hold 7 numbers under a
+//       header naming a0..a5) and 6 dihedral coefficients const, sin, cos,
+//       sin2, cos2, sin_cos (the eval's 6 terms; the header names 5).
+//   D30 heuristic dihedral: phi = +-acos(n1.n2) (dihedral/heurestic.cpp:
+//       42 writes cos) and dV/dphi exact (:55 has 2(c3 + c4) sin cos for
+//       d(c3 sin^2 + c4 cos^2)/dphi = 2(c3 - c4) sin cos).
 #include "xmd_oracle.h"
 
 #include <algorithm>
@@ -331,6 +344,14 @@ template <typename R> struct Dih {
     int i1, i2, i3, i4;
     R nat_phi;
 };
+// heur_ang / heur_dih records (angle/heurestic.h, dihedral/heurestic.h):
+// beads and the pair type 3 c1 + c2 (c: 0 GLY, 1 PRO, 2 other).
+struct HeurAngle {
+    int i1, i2, i3, type;
+};
+struct HeurDih {
+    int i1, i2, i3, i4, type;
+};
 
 // Parameters narrowed to `real`, mirroring the wiring in
 // xmd/src/freeform/main.cpp:345-907.
@@ -345,6 +366,8 @@ template <typename R> struct Params {
     R solid_eps, solid_cut, ljw_min_dist, ljw_factor, ljw_cycle, ljw_cycle_inv, vafm_H1, vafm_H2;
     std::vector<std::array<R, 6>> solid_walls, lj_walls;  // origin, normal
     R H1, H2, def_len, e_chi, ang_k, CDA, CDB, CDH;
+    bool hang_on, hdih_on;
+    R hang_c[9][7], hdih_c[9][6];  // eval_heurestic_*_forces tables by pair type
     R pauli_r_excl, pauli_depth, go_depth;
     R ces_Vf, ces_sdi, res_Vf, res_sdi, es_cutoff;
     R q[XMD_NUM_AA];
@@ -418,6 +441,8 @@ template <typename R> struct Replica {
     std::vector<Quad<R>> quads;
     std::vector<Angle<R>> angles;
     std::vector<Dih<R>> dihs;
+    std::vector<HeurAngle> hangles;
+    std::vector<HeurDih> hdihs;
     R V_last = 0;
     int64_t rebuilds = 0, steps = 0;
 };
@@ -516,6 +541,54 @@ template <typename R> void eval_snd(const Params<R> &P, Replica<R> &s, std::vect
         R diff = g.phi - d.nat_phi;
         *V += R(0.5f) * P.CDH * diff * diff;
         R dV_dphi = P.CDH * diff;
+        F[d.i1] -= dV_dphi * g.d1;
+        F[d.i2] -= dV_dphi * g.d2;
+        F[d.i3] -= dV_dphi * g.d3;
+        F[d.i4] -= dV_dphi * g.d4;
+    }
+}
+// Heuristic angles (xmd/src/forces/angle/heurestic.cpp:29-59): degree-6
+// polynomial in theta, value and derivative by one Horner sweep from the
+// top coefficient, as the reference loops d = POLY_DEG..0.
+template <typename R> void eval_heur_angle(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, size_t lo = 0, size_t hi = SIZE_MAX) {
+    for (size_t k = lo; k < std::min(hi, s.hangles.size()); ++k) {
+        auto const &a = s.hangles[k];
+        V3<R> r1 = s.r[a.i1], r2 = s.r[a.i2], r3 = s.r[a.i3];
+        V3<R> r12 = r2 - r1, r23 = r3 - r2;
+        V3<R> x12_23 = cross(r12, r23);
+        R r12_rn = norm_inv(r12), r23_rn = norm_inv(r23);
+        V3<R> dtheta_dr1 = unit(cross(r12, x12_23)) * r12_rn;
+        V3<R> dtheta_dr3 = unit(cross(r23, x12_23)) * r23_rn;
+        V3<R> dtheta_dr2 = -dtheta_dr1 - dtheta_dr3;
+        R cos_theta = -dot(r12, r23) * r12_rn * r23_rn;
+        R theta = acos_c(cos_theta);
+        R angle_V = 0, dV_dtheta = 0;
+        for (int d = 6; d >= 0; --d) {
+            R coeff = P.hang_c[a.type][d];
+            if (d > 0) dV_dtheta = (R)d * coeff + theta * dV_dtheta;
+            angle_V = coeff + theta * angle_V;
+        }
+        *V += angle_V;
+        F[a.i1] -= dV_dtheta * dtheta_dr1;
+        F[a.i2] -= dV_dtheta * dtheta_dr2;
+        F[a.i3] -= dV_dtheta * dtheta_dr3;
+    }
+}
+// Heuristic dihedrals (xmd/src/forces/dihedral/heurestic.cpp:29-70) with
+// D30: phi = +-acos(n1.n2) (the reference writes cos) and the exact
+// dV/dphi (the reference's 2(c3 + c4) sin cos term has c4's sign flipped).
+template <typename R> void eval_heur_dih(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, size_t lo = 0, size_t hi = SIZE_MAX) {
+    for (size_t k = lo; k < std::min(hi, s.hdihs.size()); ++k) {
+        auto const &d = s.hdihs[k];
+        auto g = dih_geo(s.r[d.i1], s.r[d.i2], s.r[d.i3], s.r[d.i4]);
+        V3<R> r12 = s.r[d.i2] - s.r[d.i1], r23 = s.r[d.i3] - s.r[d.i2], r34 = s.r[d.i4] - s.r[d.i3];
+        V3<R> x12_23 = cross(r12, r23), x23_34 = cross(r23, r34);
+        R cos_phi = dot(x12_23 * norm_inv(x12_23), x23_34 * norm_inv(x23_34));
+        R sin_phi = std::sin(g.phi);
+        R sin2 = sin_phi * sin_phi, cos2 = cos_phi * cos_phi, sc = sin_phi * cos_phi;
+        const R *c = P.hdih_c[d.type];
+        *V += c[0] + c[1] * sin_phi + c[2] * cos_phi + c[3] * sin2 + c[4] * cos2 + c[5] * sc;
+        R dV_dphi = c[1] * cos_phi - c[2] * sin_phi + R(2.0f) * (c[3] - c[4]) * sc + c[5] * (cos2 - sin2);
         F[d.i1] -= dV_dphi * g.d1;
         F[d.i2] -= dV_dphi * g.d2;
         F[d.i3] -= dV_dphi * g.d3;
@@ -1047,6 +1120,8 @@ template <typename R> struct System : SysBase {
         if (P.ang_on && on(XMD_TERM_NAT_ANG)) eval_angle(P, s, F, V(XMD_TERM_NAT_ANG));
         if (P.cnd_on && on(XMD_TERM_CND)) eval_cnd(P, s, F, V(XMD_TERM_CND));
         if (P.snd_on && on(XMD_TERM_SND)) eval_snd(P, s, F, V(XMD_TERM_SND));
+        if (P.hang_on && on(XMD_TERM_HEUR_ANG)) eval_heur_angle(P, s, F, V(XMD_TERM_HEUR_ANG));
+        if (P.hdih_on && on(XMD_TERM_HEUR_DIH)) eval_heur_dih(P, s, F, V(XMD_TERM_HEUR_DIH));
         if (P.pauli_on && on(XMD_TERM_PAULI)) eval_pauli(P, s, F, V(XMD_TERM_PAULI));
         if (P.go_on && on(XMD_TERM_GO)) eval_go(P, s, F, V(XMD_TERM_GO));
         if (P.ss_on && on(XMD_TERM_NAT_SS)) eval_nat_ss(P, s, F, V(XMD_TERM_NAT_SS));
@@ -1150,6 +1225,8 @@ template <typename R> struct System : SysBase {
         if (P.tether_on) { chunk(s.tethers.size(), lo, hi); range_eval(s, &Replica<R>::tethers, lo, hi, F, V, 0); }
         if (P.ang_on) { chunk(s.angles.size(), lo, hi); range_eval(s, &Replica<R>::angles, lo, hi, F, V, 1); }
         if (P.cnd_on || P.snd_on) { chunk(s.dihs.size(), lo, hi); range_eval(s, &Replica<R>::dihs, lo, hi, F, V, 2); }
+        if (P.hang_on) { chunk(s.hangles.size(), lo, hi); if (lo < hi) eval_heur_angle(P, s, F, V, lo, hi); }
+        if (P.hdih_on) { chunk(s.hdihs.size(), lo, hi); if (lo < hi) eval_heur_dih(P, s, F, V, lo, hi); }
         if (P.pauli_on) { chunk(s.pauli.size(), lo, hi); range_eval(s, &Replica<R>::pauli, lo, hi, F, V, 3); }
         if (P.go_on) { chunk(s.go_cur.size(), lo, hi); range_eval(s, &Replica<R>::go_cur, lo, hi, F, V, 4); }
         if (P.ss_on) { chunk(s.ss_cur.size(), lo, hi); range_eval(s, &Replica<R>::ss_cur, lo, hi, F, V, 9); }
@@ -1227,6 +1304,12 @@ template <typename R> int setup_params(const xmd_params *p, Params<R> &P) {
     P.ljw_on = p->lj_walls_enabled;
     P.vafm_on = p->vel_afm_enabled;
     P.fafm_on = p->force_afm_enabled;
+    P.hang_on = p->heur_ang_enabled;
+    P.hdih_on = p->heur_dih_enabled;
+    for (int k = 0; k < 9; ++k) {
+        for (int d = 0; d < 7; ++d) P.hang_c[k][d] = (R)p->heur_ang_coeffs[k][d];
+        for (int d = 0; d < 6; ++d) P.hdih_c[k][d] = (R)p->heur_dih_coeffs[k][d];
+    }
     P.solid_eps = (R)p->solid_walls_depth;
     P.solid_cut = (R)p->solid_walls_cutoff;
     P.ljw_min_dist = (R)p->lj_walls_min_dist;
@@ -1428,6 +1511,16 @@ template <typename R> int build_system(const xmd_params *p, const xmd_topology *
             if (has4 && t->nat_phi && !std::isnan(t->nat_phi[g])) s.dihs.push_back({i, i + 1, i + 2, i + 3, (R)t->nat_phi[g]});
             if (has4 && t->nat_chir && !std::isnan(t->nat_chir[g]))
                 s.quads.push_back({i, i + 1, i + 2, i + 3, (R)t->nat_chir[g], (R)t->nat_chir_factor[g]});
+            // heuristic entries (D28): model angles / dihedrals without a
+            // native value, typed by beads 1, 2 (angle) and 2, 3 (dihedral)
+            uint8_t hb = t->heur_bonded ? t->heur_bonded[g] : 0;
+            if ((hb & 1) && !has3) return fail("heuristic angle does not join consecutive residues");
+            if ((hb & 2) && !has4) return fail("heuristic dihedral does not join consecutive residues");
+            auto cls = [&](int b) { return s.atype[b] == 7 ? 0 : (s.atype[b] == 14 ? 1 : 2); };  // GLY, PRO
+            if ((hb & 1) && !(t->nat_theta && !std::isnan(t->nat_theta[g])))
+                s.hangles.push_back({i, i + 1, i + 2, 3 * cls(i) + cls(i + 1)});
+            if ((hb & 2) && !(t->nat_phi && !std::isnan(t->nat_phi[g])))
+                s.hdihs.push_back({i, i + 1, i + 2, i + 3, 3 * cls(i + 1) + cls(i + 2)});
         }
     }
     return 0;

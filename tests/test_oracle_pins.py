@@ -151,7 +151,7 @@ def synth_key(term):
     return {"tether": "tether", "chiral": "chir", "nat_ang": "nat_ang", "cnd": "cnd", "snd": "snd",
             "pauli": "pauli", "go": "go", "const_es": "const_es", "rel_es": "rel_es", "qa": "qa",
             "pid": "pid", "nat_ss": "nat_ss", "solid_wall": "solid_wall", "lj_wall": "lj_wall",
-            "vel_afm": "vel_afm", "force_afm": "force_afm"}[term]
+            "vel_afm": "vel_afm", "force_afm": "force_afm", "heur_ang": "heur_ang", "heur_dih": "heur_dih"}[term]
 
 
 @pytest.fixture(scope="module")

@@ -16,7 +16,8 @@ OK, ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_NONFINITE, ERR_STATE, ERR_NO_DEVICE = ra
 
 # xmd_term
 TERMS = ["tether", "chiral", "nat_ang", "cnd", "snd", "pauli", "go",
-         "const_es", "rel_es", "qa", "pid", "nat_ss", "solid_wall", "lj_wall", "vel_afm", "force_afm"]
+         "const_es", "rel_es", "qa", "pid", "nat_ss", "solid_wall", "lj_wall", "vel_afm", "force_afm",
+         "heur_ang", "heur_dih"]
 TERM = {name: k for k, name in enumerate(TERMS)}
 NUM_TERMS = len(TERMS)
 ALL_TERMS = (1 << NUM_TERMS) - 1
@@ -67,6 +68,8 @@ class XmdParams(C.Structure):
         ("lj_walls_min_dist", _d), ("lj_walls_breaking_factor", _d), ("lj_walls_phase_duration", _d),
         ("vel_afm_enabled", _i32), ("force_afm_enabled", _i32),
         ("vel_afm_H1", _d), ("vel_afm_H2", _d),
+        ("heur_ang_enabled", _i32), ("heur_dih_enabled", _i32),
+        ("heur_ang_coeffs", (_d * 7) * 9), ("heur_dih_coeffs", (_d * 6) * 9),
     ]
 
 
@@ -85,6 +88,7 @@ class XmdTopology(C.Structure):
         ("n_vel_afm", _i32), ("vel_afm_idx", C.POINTER(_i32)), ("vel_afm_orig", C.POINTER(_d)),
         ("vel_afm_v", C.POINTER(_d)),
         ("n_force_afm", _i32), ("force_afm_idx", C.POINTER(_i32)), ("force_afm_F", C.POINTER(_d)),
+        ("heur_bonded", C.POINTER(C.c_uint8)),
     ]
 
 

@@ -12,16 +12,18 @@ using namespace xmd_dev;
 constexpr int WAVE = 64;
 constexpr int BLK = 256;
 constexpr int MAXQ = 16;
-constexpr int NUM_TERMS = 16;
+constexpr int NUM_TERMS = 18;
 
 // F_NATSS marks a disulfide pair of the list (native_ssbonds): F_NATIVE is
 // set too, so no non-native term takes it.
 enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16, F_NATSS = 32 };
-enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4 };
+// B_HANG / B_HDIH: the model's angle (dihedral) starting at this bead has no
+// native value, so the heuristic term takes it when enabled.
+enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4, B_HANG = 8, B_HDIH = 16 };
 enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
 enum : int {
     T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID, T_NAT_SS,
-    T_SOLID_WALL, T_LJ_WALL, T_VEL_AFM, T_FORCE_AFM
+    T_SOLID_WALL, T_LJ_WALL, T_VEL_AFM, T_FORCE_AFM, T_HEUR_ANG, T_HEUR_DIH
 };
 
 // Derived parameters in double (list and QA decisions, integrator).
@@ -105,6 +107,9 @@ struct Dev {
     const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
     const int8_t *ptype;
     const uint8_t *bm_fix;
+    // heuristic bonded tables by pair type 3 c1 + c2 (c: 0 GLY, 1 PRO, 2
+    // other): 7 angle polynomial coefficients, then 6 dihedral coefficients
+    const double *heur_ang_c, *heur_dih_c;
     // rows of J^(256 2^j) (jrow), J^(256 w) for w < jwave_n (jwave) and
     // J^(4N) per replica (jadv); lane nibble table jnib
     const uint64_t *jrow, *jwave, *jnib, *jadv;

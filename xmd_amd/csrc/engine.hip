@@ -497,6 +497,8 @@ void derive(Part &E, double go_cut) {
     if (p.lj_walls_enabled) m |= 1u << T_LJ_WALL;
     if (p.vel_afm_enabled) m |= 1u << T_VEL_AFM;
     if (p.force_afm_enabled) m |= 1u << T_FORCE_AFM;
+    if (p.heur_ang_enabled) m |= 1u << T_HEUR_ANG;
+    if (p.heur_dih_enabled) m |= 1u << T_HEUR_DIH;
     E.mask = m;
     E.TTd.mask = m;
     E.TTf.mask = m;
@@ -574,6 +576,17 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
         int k = E.slot_rep_h[s];
         if (((f & B_PREV) && g - 1 < E.rep_off[k]) || ((f & B_NEXT) && g + 1 >= E.rep_off[k + 1]))
             throw Fail{XMD_ERR_ARG, "chain spans replicas"};
+        // heuristic angle / dihedral entries of the model
+        const uint8_t hb = t->heur_bonded ? t->heur_bonded[g] : 0;
+        auto same_chain = [&](int m) { return g + m < E.nbeads && chain_of[g + m] == chain_of[g]; };
+        if (hb & 1) {
+            if (!same_chain(2)) throw Fail{XMD_ERR_ARG, "heuristic angle does not join consecutive residues"};
+            f |= B_HANG;
+        }
+        if (hb & 2) {
+            if (!same_chain(3)) throw Fail{XMD_ERR_ARG, "heuristic dihedral does not join consecutive residues"};
+            f |= B_HDIH;
+        }
         bfl[s] = f;
     }
     derive(E, go_cut);
@@ -802,6 +815,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.Vterm = E.alloc<double>((size_t)NUM_TERMS * n);
     D.tether_nat = E.upload(teth);
     D.theta_nat = E.upload(th);
+    D.heur_ang_c = E.upload(std::vector<double>(&P.heur_ang_coeffs[0][0], &P.heur_ang_coeffs[0][0] + 63));
+    D.heur_dih_c = E.upload(std::vector<double>(&P.heur_dih_coeffs[0][0], &P.heur_dih_coeffs[0][0] + 54));
     D.phi_nat = E.upload(ph);
     D.chir_nat = E.upload(ch);
     D.chir_fac = E.upload(cf);
@@ -1518,6 +1533,7 @@ void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
     o.t.n_force_afm = (int32_t)o.fidx.size();
     o.t.force_afm_idx = o.fidx.data();
     o.t.force_afm_F = o.fF.data();
+    o.t.heur_bonded = t->heur_bonded ? t->heur_bonded + b0 : nullptr;
 }
 
 void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {

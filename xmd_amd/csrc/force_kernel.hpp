@@ -8,7 +8,8 @@
 namespace xmd_k {
 
 // Force on every slot from every enabled term.  LPB consecutive lanes share
-// a bead: the 8 bonded tasks (4 quads, 3 angles, the two tethers) and the
+// a bead: the 8 bonded tasks (4 quads, 3 angles, the two tethers; native or
+// heuristic angles and dihedrals by whether a native value exists) and the
 // neighbour-row entries are strided over the lanes, then a shuffle tree
 // (fixed order, hence deterministic) sums the lanes' partial forces.  The
 // result goes to Fbuf (plus the PID spill onto chain neighbours); the
@@ -70,7 +71,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
         for (int task = sub; task < 8; task += LPB) {
             if (task < 4) {
                 int a = task, first = s - 3 + a;
-                if (!(mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND)))) continue;
+                if (!(mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND) | (1u << T_HEUR_DIH)))) continue;
                 if (!in_chain(first) || !in_chain(first + 3)) continue;
                 v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2),
                            q3 = ldp(Pp, first + 3);
@@ -96,18 +97,40 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                         add3(Fx, Fy, Fz, pick4(out, 3 - a));
                         if (a == 3) addV(T_SND, (double)v);
                     }
+                } else if ((mask & (1u << T_HEUR_DIH)) && (D.bflags[first] & B_HDIH)) {
+                    // typed by the residues of beads 2 and 3
+                    const double *cd =
+                        D.heur_dih_c + 6 * (3 * heur_class(D.atype[first + 1]) + heur_class(D.atype[first + 2]));
+                    T c[6];
+                    for (int k = 0; k < 6; ++k) c[k] = (T)cd[k];
+                    T v = heur_dihedral<T>(r12, r23, r34, c, out);
+                    add3(Fx, Fy, Fz, pick4(out, 3 - a));
+                    if (a == 3) addV(T_HEUR_DIH, (double)v);
                 }
             } else if (task < 7) {
                 int a = task - 4, first = s - 2 + a;
-                if (!(mask & (1u << T_NAT_ANG))) continue;
+                if (!(mask & ((1u << T_NAT_ANG) | (1u << T_HEUR_ANG)))) continue;
                 if (!in_chain(first) || !in_chain(first + 2)) continue;
                 double th = D.theta_nat[first];
-                if (th != th) continue;
-                v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
-                v3<T> out[3];
-                T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
-                add3(Fx, Fy, Fz, pick3(out, 2 - a));
-                if (a == 2) addV(T_NAT_ANG, (double)v);
+                if (th == th) {
+                    if (!(mask & (1u << T_NAT_ANG))) continue;
+                    v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
+                    v3<T> out[3];
+                    T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
+                    add3(Fx, Fy, Fz, pick3(out, 2 - a));
+                    if (a == 2) addV(T_NAT_ANG, (double)v);
+                } else {
+                    // no native value: the heuristic angle, typed by the
+                    // residues of beads 1 and 2, FP64 in both forms
+                    if (!(mask & (1u << T_HEUR_ANG)) || !(D.bflags[first] & B_HANG)) continue;
+                    const double *ca =
+                        D.heur_ang_c + 7 * (3 * heur_class(D.atype[first]) + heur_class(D.atype[first + 1]));
+                    v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
+                    v3<double> out[3];
+                    double v = heur_angle(q1 - q0, q2 - q1, ca, out);
+                    add3(Fx, Fy, Fz, pick3(out, 2 - a));
+                    if (a == 2) addV(T_HEUR_ANG, v);
+                }
             } else {
                 // FP64 in both forms: stiff bonds cancel in r - r_nat
                 if (!(mask & (1u << T_TETHER))) continue;

@@ -4,7 +4,7 @@
 // mixed kernels narrow FP64-differenced displacements to float.
 //
 // Every function restates one reference functor body (file:line at each),
-// with the engine's decisions D1-D19 (listed in xmd_oracle.cpp's header).
+// with the engine's decisions D1-D30 (listed in xmd_oracle.cpp's header).
 // Expression order follows the reference; the file is compiled with
 // -ffp-contract=off so the FP64 decision code (lists, QA) evaluates exactly
 // as written.
@@ -489,6 +489,70 @@ XMD_HD inline T nat_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, int kind, T CDA, T
     out[3] = -(dV * d4);
     return V;
 }
+
+// Heuristic angle (angle/heurestic.cpp:29-59): V = sum_d a_d theta^d,
+// d = 0..6, with dV/dtheta by the same Horner sweep.  FP64 in both forms:
+// the coefficients alternate in sign up to ~1e5 eps/rad^d while V is a few
+// eps, so the sum cancels five digits.
+XMD_HD inline double heur_angle(v3<double> r12, v3<double> r23, const double a[7], v3<double> out[3]) {
+    v3<double> x12_23 = cross(r12, r23);
+    double r12_rn = norm_inv(r12), r23_rn = norm_inv(r23);
+    v3<double> d1 = unit(cross(r12, x12_23)) * r12_rn;
+    v3<double> d3 = unit(cross(r23, x12_23)) * r23_rn;
+    v3<double> d2 = -d1 - d3;
+    double theta = acos_c(-dot(r12, r23) * r12_rn * r23_rn);
+    double V = 0.0, dV = 0.0;
+    for (int d = 6; d >= 0; --d) {
+        if (d > 0) dV = (double)d * a[d] + theta * dV;
+        V = a[d] + theta * V;
+    }
+    out[0] = -(dV * d1);
+    out[1] = -(dV * d2);
+    out[2] = -(dV * d3);
+    return V;
+}
+// Heuristic dihedral (dihedral/heurestic.cpp:29-70): V = c0 + c1 sin phi +
+// c2 cos phi + c3 sin^2 + c4 cos^2 + c5 sin cos, phi = +-acos(n1.n2) and
+// F = -dV/dphi dphi/dr with the exact derivative (the reference writes
+// phi = cos(n1.n2) and 2(c3 + c4) sin cos in dV/dphi; decision D30).
+template <typename T>
+XMD_HD inline T heur_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, const T c[6], v3<T> out[4]) {
+    v3<T> x12_23 = cross(r12, r23), x23_34 = cross(r23, r34);
+    T x12_23_rn = norm_inv(x12_23), x23_34_rn = norm_inv(x23_34);
+    v3<T> x12_23_u = x12_23 * x12_23_rn, x23_34_u = x23_34 * x23_34_rn;
+    T cos_phi = dot(x12_23_u, x23_34_u), sin_phi;
+    T r23_n = norm(r23);
+    if constexpr (std::is_same<T, double>::value) {
+        T phi = acos_c(cos_phi);
+        if (dot(x12_23, r34) < T(0.0f)) phi = -phi;
+        sin_phi = sin(phi);
+    } else {
+        // sin phi = (x12_23 . r34) |r23| / (|x12_23| |x23_34|): signed and
+        // FP32-stable where cos phi is near +-1
+        sin_phi = dot(x12_23, r34) * r23_n * x12_23_rn * x23_34_rn;
+    }
+    T sin2 = sin_phi * sin_phi, cos2 = cos_phi * cos_phi, sc = sin_phi * cos_phi;
+    T V = c[0] + c[1] * sin_phi + c[2] * cos_phi + c[3] * sin2 + c[4] * cos2 + c[5] * sc;
+    T dV = c[1] * cos_phi - c[2] * sin_phi + T(2.0f) * (c[3] - c[4]) * sc + c[5] * (cos2 - sin2);
+    v3<T> d1 = -x12_23_u * r23_n * x12_23_rn;
+    v3<T> d4 = x23_34_u * r23_n * x23_34_rn;
+    v3<T> df_num = -d1 * dot(r12, r23) + d4 * dot(r23, r34);
+    v3<T> df;
+    if constexpr (std::is_same<T, float>::value) df = df_num * rcp_f(r23_n * r23_n);
+    else df = df_num / (r23_n * r23_n);
+    v3<T> d2 = -d1 + df;
+    v3<T> d3 = -d4 - df;
+    out[0] = -(dV * d1);
+    out[1] = -(dV * d2);
+    out[2] = -(dV * d3);
+    out[3] = -(dV * d4);
+    return V;
+}
+// Pair type of the heuristic tables (heur_ang_type, heur_dih_type):
+// 0 GLY, 1 PRO, 2 any other residue; type = 3 c1 + c2.  aa_code values of
+// GLY and PRO (xmd/include/xmd/types/amino_acid.h, alphabetical codes).
+constexpr int AA_GLY = 7, AA_PRO = 14;
+XMD_HD inline int heur_class(int aa) { return aa == AA_GLY ? 0 : (aa == AA_PRO ? 1 : 2); }
 
 // ----------------------------------------------------------------- QA ---
 // n and h of a bead with prev and next (qa/precompute_nh.cpp:12-20); zero

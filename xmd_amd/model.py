@@ -407,10 +407,12 @@ def morph_into_saw(m, gen, bond_length, residue_density, infer_box):
 def to_system(m, temperature, rng_state):
     """One-replica engine topology of a model (main.cpp:113-206,372-577):
     positions, types, chains; Go contacts = contacts other than NAT_SS at
-    their lengths; disulfide bonds = the NAT_SS contacts; per-bead native tether length, angle and dihedral where
-    the model gives a value (angles and dihedrals must join consecutive
-    residues of one chain); chirality of every model dihedral from the
-    model's positions (after morphing, as the reference sets it up)."""
+    their lengths; disulfide bonds = the NAT_SS contacts; per-bead native
+    tether length, angle and dihedral where the model gives a value, and a
+    heuristic-term bit where it gives none (angles and dihedrals must join
+    consecutive residues of one chain); chirality of every model dihedral
+    from the model's positions (after morphing, as the reference sets it
+    up)."""
     from .synth import System
 
     n = m.n_residues
@@ -433,12 +435,14 @@ def to_system(m, temperature, rng_state):
         if length is not None:
             tether[slot[i1]] = length
     theta = np.full(n, np.nan)
+    heur = np.zeros(n, dtype=np.uint8)  # angles / dihedrals without a native value
     for *ids, th in m.angles:
-        if th is None:
-            continue
         if not consecutive(ids):
             raise ValueError(f"angle {ids} does not join consecutive residues")
-        theta[slot[ids[0]]] = th
+        if th is None:
+            heur[slot[ids[0]]] |= 1
+        else:
+            theta[slot[ids[0]]] = th
     phi = np.full(n, np.nan)
     chir = np.full(n, np.nan)
     chir_f = np.full(n, np.nan)
@@ -448,6 +452,8 @@ def to_system(m, temperature, rng_state):
         s = slot[ids[0]]
         if ph is not None:
             phi[s] = ph
+        else:
+            heur[s] |= 2
         r12, r23, r34 = pos[s + 1] - pos[s], pos[s + 2] - pos[s + 1], pos[s + 3] - pos[s + 2]
         f = np.linalg.norm(r23) ** -3.0
         chir[s] = float(r12 @ np.cross(r23, r34)) * f
@@ -470,6 +476,7 @@ def to_system(m, temperature, rng_state):
         nat_i1=nat_i1, nat_i2=nat_i2, nat_dist=nat_d,
         tether_len=tether, nat_theta=theta, nat_phi=phi, nat_chir=chir, nat_chir_factor=chir_f,
         ss_i1=np.array([a for a, _ in ss], dtype=np.int32), ss_i2=np.array([b for _, b in ss], dtype=np.int32),
+        heur_bonded=heur,
     )
 
 

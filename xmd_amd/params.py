@@ -208,6 +208,44 @@ def _csv_records(node, base_dir):
     return list(csv.DictReader(io.StringIO(text)))
 
 
+def _csv_rows(node, base_dir):
+    """Raw comma-split rows after the header (the reference's readCSVRow,
+    xmd/src/files/csv.cpp:28-69, without quoting), for tables read by
+    position."""
+    text = str(node)
+    if "\n" not in text and "," not in text:
+        with open(os.path.join(base_dir, text)) as f:
+            text = f.read()
+    lines = [ln for ln in text.splitlines() if ln.strip()]
+    return [ln.split(",") for ln in lines[1:]]
+
+
+def _heur_type(c1, c2):
+    """heur_coeff_pair_type (params.cpp:58-67): G 0, P 1, anything else 2,
+    from the first character of each type field."""
+    cls = lambda f: {"G": 0, "P": 1}.get(f.strip()[:1], 2)
+    return 3 * cls(c1) + cls(c2)
+
+
+def heuristic_tables(tree, base_dir="."):
+    """Coefficient tables of "heurestic angles" / "heurestic dihedrals"
+    (params.cpp:69-124) by pair type, read by position (decision D29):
+    7 polynomial coefficients a0..a6 per angle row (eps/rad^d), 6
+    dihedral coefficients const, sin, cos, sin2, cos2, sin_cos (eps)."""
+    ang = np.zeros((9, 7))
+    dih = np.zeros((9, 6))
+    for sec, table, ncoef, unit in (("heurestic angles", ang, 7, 1.0), ("heurestic dihedrals", dih, 6, UNITS["eps"])):
+        node = (tree.get(sec) or {}).get("coefficients")
+        if node is None:
+            continue
+        for row in _csv_rows(node, base_dir):
+            vals = [f for f in row[2:] if f.strip()]
+            if len(vals) < ncoef:
+                raise ValueError(f"{sec}: row {','.join(row)} has fewer than {ncoef} coefficients")
+            table[_heur_type(row[0], row[1])] = [quantity(v.strip(), unit) for v in vals[:ncoef]]
+    return ang, dih
+
+
 def _bool(x):
     if isinstance(x, bool):
         return x
@@ -259,6 +297,14 @@ def to_xmd_params(tree, base_dir=".", precision=_abi.PREC_FP64, rng_exact=True,
     w = tree["velocity AFM"]
     p.vel_afm_enabled, p.vel_afm_H1, p.vel_afm_H2 = int(_bool(w["enabled"])), q(w["H1"]), q(w["H2"])
     p.force_afm_enabled = int(_bool(tree["force AFM"]["enabled"]))
+    p.heur_ang_enabled = int(_bool((tree.get("heurestic angles") or {}).get("enabled", False)))
+    p.heur_dih_enabled = int(_bool((tree.get("heurestic dihedrals") or {}).get("enabled", False)))
+    ang, dih = heuristic_tables(tree, base_dir)
+    for k in range(9):
+        for d in range(7):
+            p.heur_ang_coeffs[k][d] = ang[k, d]
+        for d in range(6):
+            p.heur_dih_coeffs[k][d] = dih[k, d]
     e = tree["const ES"]
     p.const_es_enabled = int(_bool(e["enabled"]))
     p.const_es_screen, p.const_es_permittivity = q(e["screening distance"]), q(e["permittivity"])

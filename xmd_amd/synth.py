@@ -131,6 +131,9 @@ class System:
     vel_afm_v: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     force_afm_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     force_afm_F: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    # per bead or None: bit 0 angle i..i+2, bit 1 dihedral i..i+3 in the
+    # model without a native value (the heuristic terms' entries)
+    heur_bonded: np.ndarray = None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -171,6 +174,7 @@ class System:
             vel_afm_v=np.ascontiguousarray(self.vel_afm_v, dtype=np.float64).reshape(-1),
             force_afm_idx=np.ascontiguousarray(self.force_afm_idx, dtype=np.int32),
             force_afm_F=np.ascontiguousarray(self.force_afm_F, dtype=np.float64).reshape(-1),
+            heur_bonded=None if self.heur_bonded is None else np.ascontiguousarray(self.heur_bonded, dtype=np.uint8),
         )
         t = _abi.XmdTopology()
         t.n_replicas = self.n_replicas
@@ -184,7 +188,7 @@ class System:
         t.n_force_afm = len(self.force_afm_idx)
         types = dict(replica_offset=C.c_int32, chain_offset=C.c_int32, atype=C.c_int8,
                      rng_state=C.c_uint64, nat_i1=C.c_int32, nat_i2=C.c_int32, ss_i1=C.c_int32, ss_i2=C.c_int32,
-                     vel_afm_idx=C.c_int32, force_afm_idx=C.c_int32)
+                     vel_afm_idx=C.c_int32, force_afm_idx=C.c_int32, heur_bonded=C.c_uint8)
         for name, a in arrs.items():
             setattr(t, name, _abi.ptr(a, types.get(name, C.c_double)))
         t._arrays = arrs  # keep alive with the struct
@@ -283,7 +287,7 @@ _FLAG = dict(go="go_enabled", pauli="pauli_enabled", tether="tether_enabled", ch
              nat_ang="nat_ang_enabled", cnd="cnd_enabled", snd="snd_enabled", const_es="const_es_enabled",
              rel_es="rel_es_enabled", qa="qa_enabled", pid="pid_enabled", nat_ss="nat_ss_enabled",
              solid_wall="solid_walls_enabled", lj_wall="lj_walls_enabled", vel_afm="vel_afm_enabled",
-             force_afm="force_afm_enabled")
+             force_afm="force_afm_enabled", heur_ang="heur_ang_enabled", heur_dih="heur_dih_enabled")
 
 
 def apply_terms(p, terms):
@@ -392,7 +396,8 @@ def replicate(system, temperatures, seeds):
         vel_afm_orig=np.concatenate([system.vel_afm_orig.reshape(-1, 3)] * k),
         vel_afm_v=np.concatenate([system.vel_afm_v.reshape(-1, 3)] * k),
         force_afm_idx=shifted(system.force_afm_idx).astype(np.int32),
-        force_afm_F=np.concatenate([system.force_afm_F.reshape(-1, 3)] * k))
+        force_afm_F=np.concatenate([system.force_afm_F.reshape(-1, 3)] * k),
+        heur_bonded=None if system.heur_bonded is None else tile(system.heur_bonded))
 
 
 def concat(systems):
@@ -417,7 +422,9 @@ def concat(systems):
         vel_afm_orig=np.concatenate([s.vel_afm_orig.reshape(-1, 3) for s in systems]),
         vel_afm_v=np.concatenate([s.vel_afm_v.reshape(-1, 3) for s in systems]),
         force_afm_idx=shift("force_afm_idx").astype(np.int32),
-        force_afm_F=np.concatenate([s.force_afm_F.reshape(-1, 3) for s in systems]))
+        force_afm_F=np.concatenate([s.force_afm_F.reshape(-1, 3) for s in systems]),
+        heur_bonded=None if all(s.heur_bonded is None for s in systems) else np.concatenate(
+            [np.zeros(s.n_beads, np.uint8) if s.heur_bonded is None else s.heur_bonded for s in systems]))
 
 
 def perturbed_positions(system, seed, sigma_A=0.3):
