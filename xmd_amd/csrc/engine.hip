@@ -1536,14 +1536,78 @@ void sub_topology(const xmd_topology *t, int r0, int r1, SubTopo &o) {
     o.t.heur_bonded = t->heur_bonded ? t->heur_bonded + b0 : nullptr;
 }
 
-void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
-    if (!p || !t) throw Fail{XMD_ERR_ARG, "null params or topology"};
+// Every precondition of xmd_params / xmd_topology that the host can check,
+// on the whole topology before it is split into stream groups and before
+// any device call (the reference throws std::runtime_error from its
+// loaders for the same kinds of input).
+void validate(const xmd_params *p, const xmd_topology *t) {
+    auto bad = [](const char *m) { throw Fail{XMD_ERR_ARG, m}; };
+    if (!p || !t) bad("null params or topology");
+    if (!(p->dt > 0)) bad("dt must be positive");
+    if (p->precision != XMD_PREC_FP64 && p->precision != XMD_PREC_MIXED) bad("bad precision");
+    if (p->force_lanes != 0 && p->force_lanes != 4 && p->force_lanes != 8 && p->force_lanes != 16)
+        bad("force_lanes must be 0, 4, 8 or 16");
+    if (p->stream_groups < 0) bad("stream_groups must be >= 0");
     if (t->n_replicas <= 0 || t->n_beads <= 0 || !t->replica_offset || !t->chain_offset || !t->r0)
-        throw Fail{XMD_ERR_ARG, "empty topology"};
-    if (t->replica_offset[0] != 0 || t->replica_offset[t->n_replicas] != t->n_beads)
-        throw Fail{XMD_ERR_ARG, "replica_offset must span [0, n_beads]"};
-    if (p->stream_groups < 0) throw Fail{XMD_ERR_ARG, "stream_groups must be >= 0"};
-    if (t->n_ssbonds < 0 || (t->n_ssbonds > 0 && (!t->ss_i1 || !t->ss_i2))) throw Fail{XMD_ERR_ARG, "bad ssbond arrays"};
+        bad("empty topology");
+    if (!t->atype || !t->box || !t->temperature || !t->rng_state) bad("null topology array");
+    const int n = t->n_beads, R = t->n_replicas;
+    const int32_t *ro = t->replica_offset, *co = t->chain_offset;
+    if (ro[0] != 0 || ro[R] != n) bad("replica_offset must span [0, n_beads]");
+    for (int r = 0; r < R; ++r)
+        if (ro[r + 1] <= ro[r]) bad("empty replica");
+    if (t->n_chains <= 0 || co[0] != 0 || co[t->n_chains] != n) bad("chain_offset must span [0, n_beads]");
+    for (int c = 0, r = 0; c < t->n_chains; ++c) {
+        if (co[c + 1] <= co[c]) bad("empty chain");
+        while (ro[r + 1] <= co[c]) ++r;
+        if (co[c + 1] > ro[r + 1]) bad("chain spans replicas");
+    }
+    for (int g = 0; g < n; ++g) {
+        if (t->atype[g] < 0 || t->atype[g] >= XMD_NUM_AA) bad("atype out of range");
+        for (int d = 0; d < 3; ++d)
+            if (!std::isfinite(t->r0[(size_t)d * n + g])) bad("non-finite initial position");
+    }
+    for (int k = 0; k < 3 * R; ++k)
+        if (!(t->box[k] >= 0) || !std::isfinite(t->box[k])) bad("box lengths must be finite and >= 0");
+    auto rep_of = [&](int g) { return (int)(std::upper_bound(ro, ro + R + 1, g) - ro) - 1; };
+    auto chain_of = [&](int g) { return (int)(std::upper_bound(co, co + t->n_chains + 1, g) - co) - 1; };
+    if (t->n_native < 0 || (t->n_native > 0 && (!t->nat_i1 || !t->nat_i2 || !t->nat_dist)))
+        bad("bad native contact arrays");
+    for (int k = 0; k < t->n_native; ++k) {
+        const int a = t->nat_i1[k], b = t->nat_i2[k];
+        if (a < 0 || b >= n || !(a < b)) bad("native contact needs 0 <= i1 < i2 < n");
+        if (rep_of(a) != rep_of(b)) bad("native contact crosses replicas");
+        if (!(t->nat_dist[k] > 0) || !std::isfinite(t->nat_dist[k])) bad("native contact distance must be positive");
+    }
+    if (t->n_ssbonds < 0 || (t->n_ssbonds > 0 && (!t->ss_i1 || !t->ss_i2))) bad("bad ssbond arrays");
+    for (int k = 0; k < t->n_ssbonds; ++k) {
+        const int a = t->ss_i1[k], b = t->ss_i2[k];
+        if (a < 0 || b >= n || !(a < b)) bad("ssbond needs 0 <= i1 < i2 < n");
+        if (rep_of(a) != rep_of(b)) bad("ssbond crosses replicas");
+    }
+    if (t->n_solid_walls < 0 || t->n_lj_walls < 0 || (t->n_solid_walls > 0 && !t->solid_walls) ||
+        (t->n_lj_walls > 0 && !t->lj_walls))
+        bad("bad wall arrays");
+    if (t->n_vel_afm < 0 || t->n_force_afm < 0 ||
+        (t->n_vel_afm > 0 && (!t->vel_afm_idx || !t->vel_afm_orig || !t->vel_afm_v)) ||
+        (t->n_force_afm > 0 && (!t->force_afm_idx || !t->force_afm_F)))
+        bad("bad AFM arrays");
+    for (int k = 0; k < t->n_vel_afm; ++k)
+        if (t->vel_afm_idx[k] < 0 || t->vel_afm_idx[k] >= n) bad("velocity AFM tip outside the beads");
+    for (int k = 0; k < t->n_force_afm; ++k)
+        if (t->force_afm_idx[k] < 0 || t->force_afm_idx[k] >= n) bad("force AFM tip outside the beads");
+    if (t->heur_bonded)
+        for (int g = 0; g < n; ++g) {
+            const uint8_t hb = t->heur_bonded[g];
+            if ((hb & 1) && !(g + 2 < n && chain_of(g + 2) == chain_of(g)))
+                bad("heuristic angle does not join consecutive residues");
+            if ((hb & 2) && !(g + 3 < n && chain_of(g + 3) == chain_of(g)))
+                bad("heuristic dihedral does not join consecutive residues");
+        }
+}
+
+void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
+    validate(p, t);
     const int R = t->n_replicas;
     const int G = std::min(R, p->stream_groups > 0 ? p->stream_groups : (R >= 2 ? 2 : 1));
     // go.cpp cutoff: lj::cutoff(nat_dist) = 2 nat_dist, max over all contacts
