@@ -86,3 +86,26 @@ def test_pdb_export_round_trip(tmp_path):
     assert [len(c) for c in back.chains] == [3, 2]
     assert [P.AA_NAMES[a] for a in back.atype] == ["MET", "GLY", "CYS", "ALA", "TRP"]
     assert np.abs(np.array(back.pos) - r.T).max() < 0.0006 * A
+
+
+def test_structure_yaml_qa_sections():
+    """The reporter's optional budget columns and QA contact section
+    (report_structure.cpp:28-51,92-141) from get_qa_state's records."""
+    from xmd_amd._host import CONTACT_DTYPE
+    m, r = small_model()
+    qa = np.zeros(2, dtype=CONTACT_DTYPE)
+    qa[0] = (0, 3, 1, 0, 0, (1, 0, 0, 0), (1, 0, 0, 0), 3.5)
+    qa[1] = (1, 4, 4 + 20 * 12 + 7, 1, 0, (0, 1, 0, 1), (0, 1, 1, 0), 12.25)
+    sync = np.array([[2, 1, 0, 1], [1, 2, 2, 0], [0, 0, 0, 0], [1, 1, 1, 1], [3, 0, 0, 0]], dtype=np.int8)
+    lines = Rp.structure_yaml(m, r, 1.5, qa_contacts=qa, sync=sync).splitlines()
+    assert lines[1] == "  idx,x[A],y[A],z[A],back,side-all,side-polar,side-hydrophobic"
+    assert lines[2] == "  0,0.000000,0.000000,0.000000,2,1,0,1"
+    k = lines.index("QA contacts: |-")
+    assert lines[k + 1] == "  idx,i1,i2,type,t0[tau],status"
+    assert lines[k + 2] == "  0,0,3,BACK_BACK,3.500000,FORMING_OR_FORMED"
+    assert lines[k + 3] == "  1,1,4,SIDE_SIDE,12.250000,BREAKING"
+    assert lines[k - 1] == "" and lines[k - 2].startswith("  1,1,4,SIDE_BACK")  # after the native contacts
+    assert lines[k + 5] == "angles: |-"
+    # without the optional parts the report is the one main.cpp writes
+    plain = Rp.structure_yaml(m, r, 1.5)
+    assert "QA contacts" not in plain and "back" not in plain.splitlines()[1]

@@ -3,8 +3,9 @@ trajectory, in the reference's formats, fed by the engine's state and its
 GPU observables (xmd_observables).
 
   report_stats      xmd/src/io/report_stats.cpp:8-41
-  report_structure  xmd/src/io/report_structure.cpp:21-215 (as main.cpp
-                    wires it: no sync or QA sections, main.cpp:338-350)
+  report_structure  xmd/src/io/report_structure.cpp:21-215 (main.cpp:330-343
+                    wires it without the sync and QA sections; both are
+                    optional arguments here)
   export_pdb        xmd/src/io/export_pdb.cpp:10-34, pdb.cpp:110-168,474-506
                     and the record writers of records.cpp / fields.cpp
 
@@ -79,12 +80,30 @@ def _csv_block(header, rows):
     return out
 
 
-def structure_yaml(m, r, active_threshold):
+_SYNC_FIELDS = ["back", "side-all", "side-polar", "side-hydrophobic"]
+_QA_TYPE = {1: "BACK_BACK", 2: "BACK_SIDE", 3: "SIDE_BACK"}  # others SIDE_SIDE (contact_type.h:14-30)
+_QA_STATUS = {0: "FORMING_OR_FORMED", 1: "BREAKING"}
+
+
+def structure_yaml(m, r, active_threshold, qa_contacts=None, sync=None):
     """The structure report of model m at positions r ((3, n), model residue
     order): residues, native contacts (current distance without periodic
-    images, active below thr 2^(-1/6) d_nat), angles, dihedrals."""
-    res = [[str(i), _f(r[0, i] / _A), _f(r[1, i] / _A), _f(r[2, i] / _A)] for i in range(r.shape[1])]
-    out = "residues: " + _csv_block(["idx", "x[A]", "y[A]", "z[A]"], res)
+    images, active below thr 2^(-1/6) d_nat), angles, dihedrals.  The
+    reporter's optional parts (report_structure.cpp:28-51,92-141), which
+    main.cpp leaves unset: `sync` ((n, 4) budgets, as from get_qa_state)
+    adds the four budget columns to the residues; `qa_contacts` (the
+    contact records of get_qa_state) adds the "QA contacts" section.  Its
+    idx is the contact's place among the live contacts (the engine keeps
+    its contact set compact, so the reference's vacant slots have no
+    counterpart)."""
+    header = ["idx", "x[A]", "y[A]", "z[A]"] + (_SYNC_FIELDS if sync is not None else [])
+    res = []
+    for i in range(r.shape[1]):
+        row = [str(i), _f(r[0, i] / _A), _f(r[1, i] / _A), _f(r[2, i] / _A)]
+        if sync is not None:
+            row += [str(int(v)) for v in np.asarray(sync).reshape(-1, 4)[i]]
+        res.append(row)
+    out = "residues: " + _csv_block(header, res)
     cut = active_threshold * 2.0 ** (-1.0 / 6.0)
     rows = []
     for k, (i1, i2, length, typ) in enumerate(m.contacts):
@@ -93,6 +112,10 @@ def structure_yaml(m, r, active_threshold):
                      "1" if d < cut * length else "0"])
     out += "\n\nnative contacts: " + _csv_block(
         ["idx", "i1", "i2", "type", "nat_dist[A]", "cur_dist[A]", "active"], rows)
+    if qa_contacts is not None:
+        rows = [[str(k), str(int(c["i1"])), str(int(c["i2"])), _QA_TYPE.get(int(c["type"]), "SIDE_SIDE"),
+                 _f(float(c["ref_time"])), _QA_STATUS[int(c["status"])]] for k, c in enumerate(qa_contacts)]
+        out += "\n\nQA contacts: " + _csv_block(["idx", "i1", "i2", "type", "t0[tau]", "status"], rows)
     rows = []
     for k, (i1, i2, i3, th) in enumerate(m.angles):
         r12, r23 = r[:, i2] - r[:, i1], r[:, i3] - r[:, i2]
@@ -120,14 +143,14 @@ class StructureReport(_Periodic):
         self.path_fmt = path_fmt
         self.thr = active_threshold
 
-    def __call__(self, t, m, r):
+    def __call__(self, t, m, r, qa_contacts=None, sync=None):
         if not self.due(t):
             return None
         path = self.path_fmt % t
         if os.path.dirname(path):
             os.makedirs(os.path.dirname(path), exist_ok=True)
         with open(path, "w") as f:
-            f.write(structure_yaml(m, r, self.thr))
+            f.write(structure_yaml(m, r, self.thr, qa_contacts, sync))
         self.last_t = t
         return path
 
