@@ -47,7 +47,7 @@ template <typename T> static void quad_all(int kind, int n, const double *geo, c
         v3<T> o[4];
         T v;
         if (kind == 2) v = chiral<T>(V<T>(g), V<T>(g + 3), V<T>(g + 6), P.e_chi, (T)nat[2 * i], (T)nat[2 * i + 1], o);
-        else v = nat_dihedral<T>(V<T>(g), V<T>(g + 3), V<T>(g + 6), kind, P.CDA, P.CDB, P.CDH, (T)nat[i], o);
+        else v = nat_dihedral<T>(dih_frame<T>(V<double>(g), V<double>(g + 3), V<double>(g + 6)), kind, P.CDA, P.CDB, P.CDH, (T)nat[i], o);
         for (int k = 0; k < 4; ++k) {
             out[13 * i + 3 * k] = o[k].x;
             out[13 * i + 3 * k + 1] = o[k].y;
@@ -62,7 +62,7 @@ template <typename T> static void angle_all(int n, const double *geo, const doub
     for (int i = 0; i < n; ++i) {
         const double *g = geo + 6 * i;
         v3<T> o[3];
-        T v = nat_angle<T>(V<T>(g), V<T>(g + 3), P.ang_k, (T)nat[i], o);
+        T v = nat_angle<T>(V<double>(g), V<double>(g + 3), P.ang_k, (T)nat[i], o);
         for (int k = 0; k < 3; ++k) {
             out[10 * i + 3 * k] = o[k].x;
             out[10 * i + 3 * k + 1] = o[k].y;

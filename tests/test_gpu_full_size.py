@@ -8,7 +8,8 @@
 * C3 (512 single chains of 30..400 beads): every replica's lists and
   per-term forces, 5 steps of the batch against the oracle's replicas;
 * C4 (400 x 250 beads, periodic): every list bit-exact and per-term forces
-  on one conformation (the oracle's O(N^2) list build is the cost).
+  on one conformation, then 5 steps of trajectory with QA state (the
+  oracle's O(N^2) list build is the cost).
 Bars as in test_gpu_parity: lists equal in order, forces within 1e-10
 relative, positions within 1e-8.
 """
@@ -107,4 +108,43 @@ def test_c4_full_box():
     ora.rebuild_lists()
     lists_equal(eng, ora, 0)
     compare_terms(eng, ora)
+    print("C4 steps", flush=True)
+    eng.step(5)
+    ora.step(5)
+    y, _, rng, t = eng.get_state(0)
+    yo, _, rngo, to = ora.get_state(0)
+    assert rng == rngo and t == to
+    assert np.abs(y - yo).max() < 1e-8
+    ce, se = eng.get_qa_state(0)
+    co, so = ora.get_qa_state(0)
+    assert len(ce) == len(co) and np.array_equal(se, so)
     eng.close()
+
+
+@pytest.mark.parametrize("cid", [2, 3, 4])
+def test_mixed_precision_full_size(cid):
+    """Mixed-precision forces within the 1e-5 rel-RMS bar of the FP64
+    engine at the full sizes, per term and in total (C2, all 512 C3
+    replicas, the 100k-bead C4 box)."""
+    from test_gpu_parity import rel_rms
+    system = synth.config_system(cid)
+    print("mixed", cid, system.n_beads, flush=True)
+    e64 = Engine(synth.config_params(cid), system)
+    emx = Engine(synth.config_params(cid, precision=_abi.PREC_MIXED), system)
+    for k in range(system.n_replicas):
+        r = _displaced(system, 3000 + k, rep=k)
+        e64.set_positions(k, r)
+        emx.set_positions(k, r)
+    for name in _abi.TERMS:
+        m = 1 << _abi.TERM[name]
+        F64, _ = e64.eval_forces(m)
+        if np.abs(F64).max() == 0:
+            continue
+        Fmx, _ = emx.eval_forces(m)
+        assert rel_rms(Fmx, F64) < 1e-5, (name, rel_rms(Fmx, F64))
+    F64, V64 = e64.eval_forces()
+    Fmx, Vmx = emx.eval_forces()
+    assert rel_rms(Fmx, F64) < 1e-5
+    assert np.allclose(Vmx.sum(axis=1), V64.sum(axis=1), rtol=1e-5, atol=1e-6)
+    e64.close()
+    emx.close()

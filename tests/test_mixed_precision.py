@@ -7,7 +7,9 @@ form, which follows the reference expressions.
   (lambda.h:30-42) in FP64;
 * every term's FP32 forces against FP64 on an evolved periodic box (4 x 250
   beads at the workload's density, oracle trajectory), total rel-RMS within
-  the 1e-5 the mixed form is held to, and the FP64 sums against the oracle.
+  the 1e-5 the mixed form is held to, and the FP64 sums against the oracle;
+* the angle and dihedral terms on the displaced 100k-bead box, whose
+  near-straight angles need the plane normals formed in FP64.
 """
 import ctypes as C
 import math
@@ -87,3 +89,38 @@ def test_mixed_force_error_on_evolved_box():
     assert rel < 1e-5, rel
     Fo, _ = ora.eval_forces(sum(1 << _abi.TERM[k] for k in Fd))
     assert math.sqrt(((tot_d - Fo.T) ** 2).sum() / (Fo ** 2).sum()) < 1e-12
+
+
+def test_bonded_terms_near_straight_angles():
+    """The 100k-bead box displaced by 0.3 A holds bond angles close to
+    straight, where the plane normals of the angle and dihedral terms
+    cancel; formed in FP64 and narrowed they keep each bonded term's FP32
+    forces within 1e-6 rel-RMS of FP64 (FP32 normals: 8e-5 for the native
+    dihedrals)."""
+    system = synth.config_system(4)
+    p = synth.config_params(4)
+    rng = np.random.default_rng(3000)
+    r = (system.r0 + rng.normal(scale=0.3 * synth.ANGSTROM, size=system.r0.shape)).T.copy()
+    L, prm, n = mixed.lib(), mixed.params_vec(p), r.shape[0]
+    chain = np.searchsorted(system.chain_offset, np.arange(n), side="right") - 1
+    q = np.flatnonzero((np.arange(n) + 3 < n) & (chain == np.roll(chain, -3)) & np.isfinite(system.nat_phi))
+    a = np.flatnonzero((np.arange(n) + 2 < n) & (chain == np.roll(chain, -2)) & np.isfinite(system.nat_theta))
+    cases = [("cnd", q, 3, lambda prec, geo, o: L.h_quad(prec, 0, len(q), mixed._p(geo), mixed._p(system.nat_phi[q]),
+                                                         mixed._p(prm), o.ctypes.data_as(F64))),
+             ("snd", q, 3, lambda prec, geo, o: L.h_quad(prec, 1, len(q), mixed._p(geo), mixed._p(system.nat_phi[q]),
+                                                         mixed._p(prm), o.ctypes.data_as(F64))),
+             ("nat_ang", a, 2, lambda prec, geo, o: L.h_angle(prec, len(a), mixed._p(geo),
+                                                              mixed._p(system.nat_theta[a]), mixed._p(prm),
+                                                              o.ctypes.data_as(F64)))]
+    for name, first, nb, call in cases:
+        geo = np.stack([r[first + k + 1] - r[first + k] for k in range(nb)], axis=1).reshape(-1)
+        F = {}
+        for prec in (0, 1):
+            o = np.zeros((len(first), 3 * (nb + 1) + 1))
+            call(prec, geo, o)
+            G = np.zeros((n, 3))
+            for k in range(nb + 1):
+                np.add.at(G, first + k, o[:, 3 * k:3 * k + 3])
+            F[prec] = G
+        rel = math.sqrt(((F[1] - F[0]) ** 2).sum() / (F[0] ** 2).sum())
+        assert rel < 1e-6, (name, rel)

@@ -75,7 +75,8 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                 if (!in_chain(first) || !in_chain(first + 3)) continue;
                 v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2),
                            q3 = ldp(Pp, first + 3);
-                v3<T> r12 = cvt<T>(q1 - q0), r23 = cvt<T>(q2 - q1), r34 = cvt<T>(q3 - q2);
+                const DihFrame<T> fr = dih_frame<T>(q1 - q0, q2 - q1, q3 - q2);
+                const v3<T> r12 = fr.r12, r23 = fr.r23, r34 = fr.r34;
                 v3<T> out[4];
                 if (mask & (1u << T_CHIRAL)) {
                     double nc = D.chir_nat[first];
@@ -88,12 +89,12 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                 double ph = D.phi_nat[first];
                 if (ph == ph) {
                     if (mask & (1u << T_CND)) {
-                        T v = nat_dihedral<T>(r12, r23, r34, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
+                        T v = nat_dihedral<T>(fr, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
                         add3(Fx, Fy, Fz, pick4(out, 3 - a));
                         if (a == 3) addV(T_CND, (double)v);
                     }
                     if (mask & (1u << T_SND)) {
-                        T v = nat_dihedral<T>(r12, r23, r34, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
+                        T v = nat_dihedral<T>(fr, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
                         add3(Fx, Fy, Fz, pick4(out, 3 - a));
                         if (a == 3) addV(T_SND, (double)v);
                     }
@@ -103,7 +104,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                         D.heur_dih_c + 6 * (3 * heur_class(D.atype[first + 1]) + heur_class(D.atype[first + 2]));
                     T c[6];
                     for (int k = 0; k < 6; ++k) c[k] = (T)cd[k];
-                    T v = heur_dihedral<T>(r12, r23, r34, c, out);
+                    T v = heur_dihedral<T>(fr, c, out);
                     add3(Fx, Fy, Fz, pick4(out, 3 - a));
                     if (a == 3) addV(T_HEUR_DIH, (double)v);
                 }
@@ -116,7 +117,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                     if (!(mask & (1u << T_NAT_ANG))) continue;
                     v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
                     v3<T> out[3];
-                    T v = nat_angle<T>(cvt<T>(q1 - q0), cvt<T>(q2 - q1), P.ang_k, (T)th, out);
+                    T v = nat_angle<T>(q1 - q0, q2 - q1, P.ang_k, (T)th, out);
                     add3(Fx, Fy, Fz, pick3(out, 2 - a));
                     if (a == 2) addV(T_NAT_ANG, (double)v);
                 } else {

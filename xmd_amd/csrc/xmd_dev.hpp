@@ -419,9 +419,15 @@ template <typename T> XMD_HD inline T chiral(v3<T> r12, v3<T> r23, v3<T> r34, T 
     out[3] = -(f * x12_23);
     return V;
 }
-// xmd/src/forces/angle/native.cpp:13-39
-template <typename T> XMD_HD inline T nat_angle(v3<T> r12, v3<T> r23, T k, T nat_theta, v3<T> out[3]) {
-    v3<T> x12_23 = cross(r12, r23);
+// xmd/src/forces/angle/native.cpp:13-39.  In FP32 the normal r12 x r23 is
+// formed in FP64 and narrowed (it cancels for a nearly straight angle, and
+// the gradient directions are built from it).
+template <typename T>
+XMD_HD inline T nat_angle(v3<double> r12d, v3<double> r23d, T k, T nat_theta, v3<T> out[3]) {
+    const v3<T> r12 = cvt<T>(r12d), r23 = cvt<T>(r23d);
+    v3<T> x12_23;
+    if constexpr (std::is_same<T, double>::value) x12_23 = cross(r12, r23);
+    else x12_23 = cvt<T>(cross(r12d, r23d));
     T r12_rn = norm_inv(r12), r23_rn = norm_inv(r23);
     v3<T> d1 = unit(cross(r12, x12_23)) * r12_rn;
     v3<T> d3 = unit(cross(r23, x12_23)) * r23_rn;
@@ -441,20 +447,48 @@ template <typename T> XMD_HD inline T nat_angle(v3<T> r12, v3<T> r23, T k, T nat
     out[2] = -(dV * d3);
     return V;
 }
+// The bonds of a dihedral (from FP64 displacements), its two plane normals
+// x12_23 = r12 x r23, x23_34 = r23 x r34 and the triple product x12_23.r34.
+// In FP32 the normals and the triple product are formed in FP64 and then
+// narrowed: for a nearly straight angle the cross product cancels, and the
+// dihedral's gradient scales with 1/|x| (FP32 normals put the 100k-bead
+// box's dihedral forces 5e-5 off FP64).  The FP64 form is the reference's
+// expressions.
+template <typename T> struct DihFrame {
+    v3<T> r12, r23, r34, x12_23, x23_34;
+    T trip;
+};
+template <typename T> XMD_HD inline DihFrame<T> dih_frame(v3<double> r12d, v3<double> r23d, v3<double> r34d) {
+    DihFrame<T> f;
+    f.r12 = cvt<T>(r12d);
+    f.r23 = cvt<T>(r23d);
+    f.r34 = cvt<T>(r34d);
+    if constexpr (std::is_same<T, double>::value) {
+        f.x12_23 = cross(f.r12, f.r23);
+        f.x23_34 = cross(f.r23, f.r34);
+        f.trip = dot(f.x12_23, f.r34);
+    } else {
+        const v3<double> a = cross(r12d, r23d), b = cross(r23d, r34d);
+        f.x12_23 = cvt<T>(a);
+        f.x23_34 = cvt<T>(b);
+        f.trip = (T)dot(a, r34d);
+    }
+    return f;
+}
 // complex_native.cpp:13-49 (kind 0) / simple_native.cpp:13-45 (kind 1)
 template <typename T>
-XMD_HD inline T nat_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, int kind, T CDA, T CDB, T CDH, T nat_phi, v3<T> out[4]) {
-    v3<T> x12_23 = cross(r12, r23), x23_34 = cross(r23, r34);
+XMD_HD inline T nat_dihedral(const DihFrame<T> &F, int kind, T CDA, T CDB, T CDH, T nat_phi, v3<T> out[4]) {
+    const v3<T> r12 = F.r12, r23 = F.r23, r34 = F.r34, x12_23 = F.x12_23, x23_34 = F.x23_34;
     T x12_23_rn = norm_inv(x12_23), x23_34_rn = norm_inv(x23_34);
     v3<T> x12_23_u = x12_23 * x12_23_rn, x23_34_u = x23_34 * x23_34_rn;
     T phi;
     if constexpr (std::is_same<T, double>::value) {
         T cos_phi = dot(x12_23_u, x23_34_u);
         phi = acos_c(cos_phi);
-        if (dot(x12_23, r34) < T(0.0f)) phi = -phi;
+        if (F.trip < T(0.0f)) phi = -phi;
     } else {
         // same angle, FP32-stable: (x12_23 x x23_34) = (x12_23.r34) r23
-        phi = atan2_f(dot(x12_23, r34) * norm(r23), dot(x12_23, x23_34));
+        phi = atan2_f(F.trip * norm(r23), dot(x12_23, x23_34));
     }
     T diff = phi - nat_phi;
     T V, dV;
@@ -516,20 +550,20 @@ XMD_HD inline double heur_angle(v3<double> r12, v3<double> r23, const double a[7
 // F = -dV/dphi dphi/dr with the exact derivative (the reference writes
 // phi = cos(n1.n2) and 2(c3 + c4) sin cos in dV/dphi; decision D30).
 template <typename T>
-XMD_HD inline T heur_dihedral(v3<T> r12, v3<T> r23, v3<T> r34, const T c[6], v3<T> out[4]) {
-    v3<T> x12_23 = cross(r12, r23), x23_34 = cross(r23, r34);
+XMD_HD inline T heur_dihedral(const DihFrame<T> &F, const T c[6], v3<T> out[4]) {
+    const v3<T> r12 = F.r12, r23 = F.r23, r34 = F.r34, x12_23 = F.x12_23, x23_34 = F.x23_34;
     T x12_23_rn = norm_inv(x12_23), x23_34_rn = norm_inv(x23_34);
     v3<T> x12_23_u = x12_23 * x12_23_rn, x23_34_u = x23_34 * x23_34_rn;
     T cos_phi = dot(x12_23_u, x23_34_u), sin_phi;
     T r23_n = norm(r23);
     if constexpr (std::is_same<T, double>::value) {
         T phi = acos_c(cos_phi);
-        if (dot(x12_23, r34) < T(0.0f)) phi = -phi;
+        if (F.trip < T(0.0f)) phi = -phi;
         sin_phi = sin(phi);
     } else {
         // sin phi = (x12_23 . r34) |r23| / (|x12_23| |x23_34|): signed and
         // FP32-stable where cos phi is near +-1
-        sin_phi = dot(x12_23, r34) * r23_n * x12_23_rn * x23_34_rn;
+        sin_phi = F.trip * r23_n * x12_23_rn * x23_34_rn;
     }
     T sin2 = sin_phi * sin_phi, cos2 = cos_phi * cos_phi, sc = sin_phi * cos_phi;
     T V = c[0] + c[1] * sin_phi + c[2] * cos_phi + c[3] * sin2 + c[4] * cos2 + c[5] * sc;
