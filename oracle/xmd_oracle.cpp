@@ -20,6 +20,9 @@
 //       Debye-Hueckel filters every nl pair (the filters read a
 //       nonexistent nl->particle_pairs, pauli.cpp:15, es_pair.cpp:11,
 //       bundle.cpp:11, free_pair.cpp:12).
+//   D3  lang_pc's dt, r and num_particles are never assigned and its
+//       mass arrays point at temporaries (main.cpp:208-253,217-225): dt
+//       from integrator.dt, r = y0, N = the bead count, every mass 1.
 //   D4  QA runs precompute_nh -> sift -> process_candidates ->
 //       process_contacts every step (main.cpp:1040-1042 runs only two).
 //   D5  QA contact LJ: BB -> bb, BS -> bs, SB -> sb, SS(a1,a2) ->
@@ -40,6 +43,9 @@
 //   D13 noise amplitude sqrt(2 kB T gamma / m) without dt^-1/2, as written
 //       (lang_pc.cpp:7,13,19).
 //   D14 noise is float Box-Muller in every precision mode.
+//   D15 synthetic natives use the evaluators' angle convention,
+//       theta = acos(-u12.u23) (angle/native.cpp:29-30); models read from
+//       PDB files keep the reference's loader convention (D21).
 //   D16 chirality force on bead 1 is f*(r23 x r34) (chiral.cpp:33 uses
 //       r12 x r23, which is not -dV/dr1 and breaks sum F = 0).
 //   D17 beads without prev/next keep n = h = 0 (never written,
