@@ -290,10 +290,24 @@ def brute_force_lists(system, p, rep=0):
     return out
 
 
-@pytest.mark.parametrize("cid", [1, 2])
+def odd_box_system(kind):
+    """4 x 60 beads of the C2 density in a non-cubic periodic box
+    (100 x 134 x 180 A) or a slab periodic in x and y only (z open)."""
+    import dataclasses
+    system = synth.config_system(2, chains=4, chain_len=60)
+    L = system.box[0, 0]
+    box = np.array([[100.0 * synth.ANGSTROM, L, 180.0 * synth.ANGSTROM]]) if kind == "ortho" \
+        else np.array([[L, L, 0.0]])
+    return dataclasses.replace(system, box=box)
+
+
+@pytest.mark.parametrize("cid", [1, 2, "ortho", "slab"])
 def test_lists_match_brute_force(cid):
-    system = synth.config_system(1, chain_len=150) if cid == 1 else synth.config_system(2, chains=4, chain_len=60)
-    p = synth.config_params(cid)
+    if cid in ("ortho", "slab"):
+        system = odd_box_system(cid)
+    else:
+        system = synth.config_system(1, chain_len=150) if cid == 1 else synth.config_system(2, chains=4, chain_len=60)
+    p = synth.config_params(2 if cid in ("ortho", "slab") else cid)
     ora = Oracle(p, system)
     ora.rebuild_lists()
     ref = brute_force_lists(system, p)
