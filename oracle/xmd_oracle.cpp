@@ -74,6 +74,11 @@
 //   D24 disulfide pairs of the list (native_ssbonds) belong to neither
 //       native_contacts nor non_native, so no pair term but the bond sees
 //       them (Debye-Hueckel reads native_contacts + non_native, D2).
+//   D31 both Debye-Hueckel variants enabled: main.cpp:606-645 builds one
+//       pair list with the constant variant's update (cutoff 2 screening
+//       distance) and adds each variant's cutoff to max_cutoff; the
+//       evaluators' pair pointer is never bound there, so both are taken
+//       to run over that list (both are called, main.cpp:1038-1039).
 //   D28 heuristic angles and dihedrals: main.cpp never creates their
 //       kernels, so their lists are defined here as the model's angles and
 //       dihedrals without a native value (the topology's heur_bonded
@@ -1403,7 +1408,8 @@ template <typename R> int build_system(const xmd_params *p, const xmd_topology *
     if (P.pauli_on) mc = std::max(mc, P.pauli_r_excl);
     if (P.go_on)
         for (int k = 0; k < t->n_native; ++k) mc = std::max(mc, (R)(2.0 * (double)(R)t->nat_dist[k]));
-    if (P.ces_on || P.res_on) mc = std::max(mc, P.es_cutoff);
+    if (P.ces_on) mc = std::max(mc, (R)(2.0 * p->const_es_screen));  // main.cpp:628
+    if (P.res_on) mc = std::max(mc, (R)(2.0 * p->rel_es_screen));    // main.cpp:642
     if (P.qa_on) mc = std::max(mc, P.max_formation_min_dist);
     if (P.pid_on) mc = std::max(mc, P.pid_cutoff);
     if (P.ss_on) mc = std::max(mc, R(2.0) * P.ss_equil);  // main.cpp:592-593

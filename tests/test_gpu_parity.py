@@ -512,3 +512,47 @@ def test_odd_boxes(kind, prec):
         assert len(ce) == len(co) and np.array_equal(se, so)
         lists_equal(eng, ora, 0)
     eng.close()
+
+
+def _variant_params(variant, precision=_abi.PREC_FP64):
+    p = synth.config_params(2, precision=precision)
+    if variant in ("rel_es", "both_es"):
+        p.rel_es_enabled = 1
+        p.const_es_enabled = int(variant == "both_es")
+    elif variant == "algebraic":
+        p.pid_lambda_version = _abi.LAMBDA_ALGEBRAIC
+    elif variant == "no_sink":
+        p.ss_sinking = 0
+    return p
+
+
+@pytest.mark.parametrize("variant", ["rel_es", "both_es", "algebraic", "no_sink"])
+def test_term_variants(variant, c2_small):
+    """Parameter variants of the periodic full-term box: relative
+    Debye-Hueckel alone and beside the constant one (D31), the algebraic
+    PID lambda, plain LJ for the side-side variants (no sink): lists,
+    per-term forces and 10 steps against the oracle in FP64, forces within
+    1e-5 in mixed precision."""
+    _, system = c2_small
+    print("variant", variant, flush=True)
+    eng, ora = Engine(_variant_params(variant), system), Oracle(_variant_params(variant), system)
+    emx = Engine(_variant_params(variant, _abi.PREC_MIXED), system)
+    r = synth.perturbed_positions(system, 9)
+    for e in (eng, ora, emx):
+        e.set_positions(0, r)
+        e.rebuild_lists()
+    lists_equal(eng, ora, 0)
+    compare_terms(eng, ora)
+    F, _ = emx.eval_forces()
+    Fo, _ = ora.eval_forces()
+    assert rel_rms(F, Fo) < 1e-5
+    eng.step(10)
+    ora.step(10)
+    y, _, rng, t = eng.get_state(0)
+    yo, _, rngo, to = ora.get_state(0)
+    assert rng == rngo and t == to and np.abs(y - yo).max() < 1e-9
+    ce, se = eng.get_qa_state(0)
+    co, so = ora.get_qa_state(0)
+    assert len(ce) == len(co) and np.array_equal(se, so)
+    eng.close()
+    emx.close()

@@ -555,3 +555,25 @@ def test_parallel_list_build_equals_serial():
         y1, y2 = b.get_list(0, name)
         assert np.array_equal(x1, y1) and np.array_equal(x2, y2), name
     assert len(a.get_list(0, "nl_all")[0]) > 10000
+
+
+def test_both_debye_hueckel_variants_share_the_list(small_open):
+    """D31: with both Debye-Hueckel variants enabled (equal screening
+    distances, so the list is the same as with either alone) each term's
+    energy and force equal its run alone: both evaluators run over the one
+    list."""
+    r = _compressed(small_open, 0.8)
+    res = {}
+    for name, terms in (("const", {"const_es": True}), ("rel", {"rel_es": True}),
+                        ("both", {"const_es": True, "rel_es": True})):
+        p = _single_term(1, "tether")
+        synth.apply_terms(p, dict(tether=False, **terms))
+        ora = Oracle(p, small_open)
+        ora.set_positions(0, r)
+        ora.rebuild_lists()
+        res[name] = {t: ora.eval_forces(1 << _abi.TERM[t]) for t in ("const_es", "rel_es")}
+    kc, kr = _abi.TERM["const_es"], _abi.TERM["rel_es"]
+    assert res["both"]["const_es"][1][0, kc] == res["const"]["const_es"][1][0, kc] != 0
+    assert res["both"]["rel_es"][1][0, kr] == res["rel"]["rel_es"][1][0, kr] != 0
+    assert np.array_equal(res["both"]["rel_es"][0], res["rel"]["rel_es"][0])
+    assert np.array_equal(res["both"]["const_es"][0], res["const"]["const_es"][0])

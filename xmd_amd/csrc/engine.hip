@@ -403,7 +403,10 @@ void derive(Part &E, double go_cut) {
     double mc = 0.0;
     if (p.pauli_enabled) mc = std::max(mc, p.pauli_r_excl);
     if (p.go_enabled) mc = std::max(mc, go_cut);
-    if (p.const_es_enabled || p.rel_es_enabled) mc = std::max(mc, es_cut);
+    // each enabled variant adds its own cutoff (main.cpp:628,642); the
+    // pair list keeps the constant variant's (D31)
+    if (p.const_es_enabled) mc = std::max(mc, 2.0 * p.const_es_screen);
+    if (p.rel_es_enabled) mc = std::max(mc, 2.0 * p.rel_es_screen);
     if (p.qa_enabled) mc = std::max(mc, mfmd);
     if (p.pid_enabled) mc = std::max(mc, pid_cut);
     // native_ss.cpp: harmonic::cutoff(equilibrium distance) = 2 * it
@@ -459,12 +462,12 @@ void derive(Part &E, double go_cut) {
         Pt.pauli_r_excl = (T)p.pauli_r_excl;
         Pt.pauli_depth = (T)p.pauli_depth;
         Pt.go_depth = (T)p.go_depth;
-        Pt.es_relative = p.const_es_enabled ? 0 : 1;
-        double Vf = p.const_es_enabled ? 1.0 / (4.0 * M_PI * p.const_es_permittivity)
-                                       : 1.0 / (4.0 * M_PI * p.rel_es_factor);
-        double sdi = p.const_es_enabled ? 1.0 / p.const_es_screen : 1.0 / p.rel_es_screen;
-        Pt.es_Vf = (T)Vf;
-        Pt.es_sdi = (T)sdi;
+        // main.cpp:618-642: V_factor = 1/(4 pi eps), screen_dist_inv per variant
+        Pt.es_relative = 0;
+        Pt.es_Vf = (T)(1.0 / (4.0 * M_PI * p.const_es_permittivity));
+        Pt.es_sdi = (T)(1.0 / p.const_es_screen);
+        Pt.res_Vf = (T)(1.0 / (4.0 * M_PI * p.rel_es_factor));
+        Pt.res_sdi = (T)(1.0 / p.rel_es_screen);
         Pt.bbp_psi0 = (T)p.pid_bbp_psi0;
         Pt.bbp_alpha = (T)p.pid_bbp_alpha;
         Pt.bbp_rmin = (T)p.pid_bbp_rmin;
@@ -489,7 +492,7 @@ void derive(Part &E, double go_cut) {
     if (p.pauli_enabled) m |= 1u << T_PAULI;
     if (p.go_enabled) m |= 1u << T_GO;
     if (p.const_es_enabled) m |= 1u << T_CONST_ES;
-    else if (p.rel_es_enabled) m |= 1u << T_REL_ES;
+    if (p.rel_es_enabled) m |= 1u << T_REL_ES;
     if (p.qa_enabled) m |= 1u << T_QA;
     if (p.pid_enabled) m |= 1u << T_PID;
     if (p.nat_ss_enabled) m |= 1u << T_NAT_SS;

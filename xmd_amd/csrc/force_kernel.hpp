@@ -175,8 +175,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
         if (rmask && !(D.err[0] & E_NL_CAP)) {
             const int len = D.nl_len[s];
             const int base = D.slice_start[s >> 6] + (s & 63);
-            const int es_term = P.es_relative ? T_REL_ES : T_CONST_ES;
-            const bool es_on = mask & (1u << es_term);
+            const bool ces_on = mask & (1u << T_CONST_ES), res_on = mask & (1u << T_REL_ES);
             const T q_s = (T)D.charge[D.atype[s]];
             const v3<double> rsp = (bf & B_PREV) ? ldp(Pp, s - 1) : ri;
             const v3<double> rsn = (bf & B_NEXT) ? ldp(Pp, s + 1) : ri;
@@ -194,11 +193,19 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                     add3(Fx, Fy, Fz, f.F);
                     if (owner) addV(T_PAULI, (double)f.V);
                 }
-                if ((fl & F_ES) && es_on) {
+                if ((fl & F_ES) && (ces_on || res_on)) {
+                    // both variants run over the one list when both are on (D31)
                     T q1q2 = owner ? q_s * (T)D.charge[D.atype[j]] : (T)D.charge[D.atype[j]] * q_s;
-                    FV<T> f = es_pair<T>(d, q1q2, P.es_Vf, P.es_sdi, P.es_relative);
-                    add3(Fx, Fy, Fz, f.F);
-                    if (owner) addV(es_term, (double)f.V);
+                    if (ces_on) {
+                        FV<T> f = es_pair<T>(d, q1q2, P.es_Vf, P.es_sdi, 0);
+                        add3(Fx, Fy, Fz, f.F);
+                        if (owner) addV(T_CONST_ES, (double)f.V);
+                    }
+                    if (res_on) {
+                        FV<T> f = es_pair<T>(d, q1q2, P.res_Vf, P.res_sdi, 1);
+                        add3(Fx, Fy, Fz, f.F);
+                        if (owner) addV(T_REL_ES, (double)f.V);
+                    }
                 }
                 if ((fl & F_PID) && (mask & (1u << T_PID))) {
                     // The bundle from this bead's side: s in the first slot.

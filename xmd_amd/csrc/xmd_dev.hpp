@@ -217,8 +217,12 @@ template <typename T> XMD_HD inline VdV<T> lam_eval(T psi0, T alpha, int version
 template <typename T> struct TermParams {
     T H1, H2, e_chi, ang_k, CDA, CDB, CDH;
     T pauli_r_excl, pauli_depth, go_depth;
+    // Debye-Hueckel: es_* with es_relative picking the variant (0 const,
+    // 1 relative); the engine keeps the constant variant there and the
+    // relative one in res_*, so that both run when both are enabled
     T es_Vf, es_sdi;
     int es_relative;
+    T res_Vf, res_sdi;
     T bbp_psi0, bbp_alpha, bbp_rmin, bbp_depth;
     T bbm_psi0, bbm_alpha, bbm_rmin, bbm_depth;
     T ss_psi0, ss_alpha;
