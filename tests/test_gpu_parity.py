@@ -556,3 +556,30 @@ def test_term_variants(variant, c2_small):
     assert len(ce) == len(co) and np.array_equal(se, so)
     eng.close()
     emx.close()
+
+
+def test_c0_cutoff_override():
+    """C0: the 60-bead chain with the list radius fixed at 18 A + 2 A pad
+    (max_cutoff_override) instead of the term cutoffs: lists, forces and a
+    200-step trajectory against the oracle."""
+    system = synth.config_system(0)
+    params = synth.config_params(0)
+    assert params.max_cutoff_override > 0
+    eng, ora = Engine(params, system), Oracle(params, system)
+    r = synth.perturbed_positions(system, 3)
+    eng.set_positions(0, r)
+    ora.set_positions(0, r)
+    eng.rebuild_lists()
+    ora.rebuild_lists()
+    lists_equal(eng, ora, 0)
+    compare_terms(eng, ora)
+    # rebuilds during the steps (the engine also counts the one made when
+    # it was created)
+    rb0, rbo0 = eng.counters()["rebuilds"], ora.counters()["rebuilds"]
+    eng.step(200)
+    ora.step(200)
+    y, _, rng, t = eng.get_state(0)
+    yo, _, rngo, to = ora.get_state(0)
+    assert rng == rngo and t == to and np.abs(y - yo).max() < 1e-8
+    assert eng.counters()["rebuilds"] - rb0 == ora.counters()["rebuilds"] - rbo0 > 0
+    eng.close()
