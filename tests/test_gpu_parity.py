@@ -583,3 +583,35 @@ def test_c0_cutoff_override():
     assert rng == rngo and t == to and np.abs(y - yo).max() < 1e-8
     assert eng.counters()["rebuilds"] - rb0 == ora.counters()["rebuilds"] - rbo0 > 0
     eng.close()
+
+
+def test_verlet_list_complete_between_rebuilds():
+    """Between rebuilds the Verlet list must hold every pair closer than
+    the list cutoff (nl::verify's skin criterion, verify.cpp:10-21): a
+    4-box batch of the periodic full-term workload (10k beads each) checked
+    against a k-d tree every 100 of 2000 steps."""
+    from scipy.spatial import cKDTree
+    batch = synth.config_system(2, n_batch=4, horizon_steps=2000)
+    p = synth.config_params(2, precision=_abi.PREC_MIXED)
+    ss = np.array(p.ss_rmax)
+    mc = max(p.pauli_r_excl, (2.0 * batch.nat_dist).max(), 2.0 * p.const_es_screen,
+             max(p.lj_bb_rmin, p.lj_bs_rmin, ss.max()), max(2 * p.lj_bb_rmin, 2 * p.lj_bs_rmin, (2 * ss).max()))
+    eng = Engine(p, batch)
+    rebuilds = eng.counters()["rebuilds"]
+    for step in range(100, 2001, 100):
+        eng.step(100)
+        for k in range(batch.n_replicas):
+            sl = batch.replica_slice(k)
+            y = eng.get_state(k)[0][0].T
+            L = batch.box[k]
+            tree = cKDTree(y - L * np.floor(y / L), boxsize=L * (1 + 1e-12))
+            pr = tree.query_pairs(mc * (1 - 1e-9), output_type="ndarray")
+            i, j = pr.min(axis=1), pr.max(axis=1)
+            chain = np.searchsorted(batch.chain_offset, np.arange(sl.start, sl.stop), side="right") - 1
+            keep = ~((chain[i] == chain[j]) & (j - i < 3))
+            a, b = eng.get_list(k, "nl_all")
+            have = set(zip(a.tolist(), b.tolist()))
+            missing = [q for q in zip(i[keep].tolist(), j[keep].tolist()) if q not in have]
+            assert not missing, (step, k, missing[:5])
+    assert eng.counters()["rebuilds"] > rebuilds
+    eng.close()
