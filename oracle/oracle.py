@@ -14,13 +14,13 @@ from xmd_amd import _abi
 from xmd_amd._host import HostSystem
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_LIBS = {}
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+def load(path):
+    """The oracle library at `path` (loaded once per path)."""
+    if path not in _LIBS:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle` or __graft_entry__.build()")
         L = C.CDLL(path)
@@ -34,34 +34,40 @@ def lib():
         L.oracle_create.argtypes = [C.POINTER(_abi.XmdParams), C.POINTER(_abi.XmdTopology), C.c_int, C.c_void_p]
         L.oracle_destroy.argtypes = [C.c_void_p]
         L.oracle_step.argtypes = [C.c_void_p, C.c_int64, C.c_int]
-        _LIB = L
-    return _LIB
+        _LIBS[path] = L
+    return _LIBS[path]
+
+
+def lib():
+    return load(LIB_PATH)
 
 
 class Oracle(HostSystem):
-    """FP64 (real_kind=0) or float (real_kind=1) CPU oracle of a system."""
+    """FP64 (real_kind=0) or float (real_kind=1) CPU oracle of a system;
+    lib_path selects another build of the oracle (the mutant builds of the
+    value-pin tests)."""
 
-    def __init__(self, params, system, real_kind=0):
+    def __init__(self, params, system, real_kind=0, lib_path=None):
         super().__init__(params, system)
-        L = lib()
+        self._lib = load(lib_path or LIB_PATH)
         h = C.c_void_p()
-        rc = L.oracle_create(C.byref(params), C.byref(self.topo), int(real_kind), C.byref(h))
+        rc = self._lib.oracle_create(C.byref(params), C.byref(self.topo), int(real_kind), C.byref(h))
         if rc != 0:
-            raise RuntimeError(L.oracle_last_error().decode())
+            raise RuntimeError(self._lib.oracle_last_error().decode())
         self.handle = h
 
     def _fn(self, name):
-        return getattr(lib(), "oracle_" + name)
+        return getattr(self._lib, "oracle_" + name)
 
     def _err(self):
-        return lib().oracle_last_error().decode()
+        return self._lib.oracle_last_error().decode()
 
     def step(self, nsteps, nthreads=0):
-        self._check(lib().oracle_step(self.handle, C.c_int64(nsteps), int(nthreads)))
+        self._check(self._lib.oracle_step(self.handle, C.c_int64(nsteps), int(nthreads)))
 
     def close(self):
         if self.handle:
-            lib().oracle_destroy(self.handle)
+            self._lib.oracle_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

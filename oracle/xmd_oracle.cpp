@@ -94,6 +94,22 @@
 //       d(c3 sin^2 + c4 cos^2)/dphi = 2(c3 - c4) sin cos).
 #include "xmd_oracle.h"
 
+// ORACLE_MUTANT plants one known mistake, for test builds only (the
+// shipped oracle is built with 0): tests/test_oracle_value_pins.py builds
+// each mutant in a temporary directory and checks that the closed-form
+// value pins reject it.  Every mutant keeps F = -grad V, so only a pin of
+// V itself can catch it.
+//   1  PID: bb+ lambda paired with the bb- LJ and vice versa
+//   2  PID: side-side table index transposed (20 a2 + a1)
+//   3  PID: sign of psi flipped (with its gradient)
+//   4  native angle: the supplement pi - theta (with its gradient)
+//   5  native dihedrals: sign of phi flipped (with its gradient)
+//   6  chirality: sign of the triple product flipped (with its gradient)
+//   7  relative Debye-Hueckel: e^(-r/s)/r instead of e^(-r/s)/r^2
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -479,9 +495,11 @@ template <typename R> void eval_chiral(const Params<R> &P, Replica<R> &s, std::v
         V3<R> r12 = r2 - r1, r23 = r3 - r2, r34 = r4 - r3;
         V3<R> x12_23 = cross(r12, r23), x12_34 = cross(r12, r34), x23_34 = cross(r23, r34);
         R chir = dot(r12, x23_34) * q.nat_factor;
+        if (ORACLE_MUTANT == 6) chir = -chir;
         R chir_diff = chir - q.nat_chir;
         *V += R(0.5f) * P.e_chi * chir_diff * chir_diff;
         R f = P.e_chi * chir_diff * q.nat_factor;
+        if (ORACLE_MUTANT == 6) f = -f;
         F[q.i1] += f * x23_34;
         F[q.i2] -= f * (x12_34 + x23_34);
         F[q.i3] += f * (x12_23 + x12_34);
@@ -500,9 +518,11 @@ template <typename R> void eval_angle(const Params<R> &P, Replica<R> &s, std::ve
         V3<R> dtheta_dr2 = -dtheta_dr1 - dtheta_dr3;
         R cos_theta = -dot(r12, r23) * r12_rn * r23_rn;
         R theta = acos_c(cos_theta);
+        if (ORACLE_MUTANT == 4) theta = R(M_PI) - theta;
         R dtheta = theta - a.nat_theta;
         *V += R(0.5) * P.ang_k * dtheta * dtheta;
         R dV_dtheta = P.ang_k * dtheta;
+        if (ORACLE_MUTANT == 4) dV_dtheta = -dV_dtheta;
         F[a.i1] -= dV_dtheta * dtheta_dr1;
         F[a.i2] -= dV_dtheta * dtheta_dr2;
         F[a.i3] -= dV_dtheta * dtheta_dr3;
@@ -528,6 +548,7 @@ template <typename R> DihGeo<R> dih_geo(V3<R> r1, V3<R> r2, V3<R> r3, V3<R> r4) 
     V3<R> df = (-dphi_dr1 * dot(r12, r23) + dphi_dr4 * dot(r23, r34)) / (r23_n * r23_n);
     V3<R> dphi_dr2 = -dphi_dr1 + df;
     V3<R> dphi_dr3 = -dphi_dr4 - df;
+    if (ORACLE_MUTANT == 5) return {-phi, -dphi_dr1, -dphi_dr2, -dphi_dr3, -dphi_dr4};
     return {phi, dphi_dr1, dphi_dr2, dphi_dr3, dphi_dr4};
 }
 template <typename R> void eval_cnd(const Params<R> &P, Replica<R> &s, std::vector<V3<R>> &F, R *V, size_t lo = 0, size_t hi = SIZE_MAX) {
@@ -732,6 +753,10 @@ template <typename R> void eval_es(const Params<R> &P, Replica<R> &s, std::vecto
         if (relative) {
             Vij = Vf * e.q1_x_q2 * std::exp(-r12_n * sdi) * r12_rn * r12_rn;
             dVij_dr = -Vij * (sdi + R(2.0f) * r12_rn);
+            if (ORACLE_MUTANT == 7) {
+                Vij = Vf * e.q1_x_q2 * std::exp(-r12_n * sdi) * r12_rn;
+                dVij_dr = -Vij * (sdi + r12_rn);
+            }
         } else {
             Vij = Vf * e.q1_x_q2 * std::exp(-r12_n * sdi) * r12_rn;
             dVij_dr = -Vij * (sdi + r12_rn);
@@ -771,6 +796,7 @@ template <typename R> PsiGeo<R> pid_psi(const Box<R> &box, V3<R> r1_, V3<R> r2_,
     R p = dot(rij, rkj) * inv2, q = dot(rkl, rkj) * inv2;
     V3<R> gj = (p - R(1)) * gi - q * gl;
     V3<R> gk = (q - R(1)) * gl - p * gi;
+    if (ORACLE_MUTANT == 3) return {-psi, -gk, -gi, -gj, -gl};
     return {psi, gk, gi, gj, gl};
 }
 
@@ -1010,6 +1036,7 @@ template <typename R> void update_lists(const Params<R> &P, Replica<R> &s) {
             if (norm_inv(s.box.r_uv(s.r[p.i1], s.r[p.i2])) > min_norm_inv) {
                 if (s.prev[p.i1] < 0 || s.next[p.i1] < 0 || s.prev[p.i2] < 0 || s.next[p.i2] < 0) continue;
                 int16_t type = (int16_t)(s.atype[p.i1] * XMD_NUM_AA + s.atype[p.i2]);
+                if (ORACLE_MUTANT == 2) type = (int16_t)(s.atype[p.i2] * XMD_NUM_AA + s.atype[p.i1]);
                 s.pid.push_back({p.i1, p.i2, type});
             }
         }
@@ -1365,6 +1392,7 @@ template <typename R> int setup_params(const xmd_params *p, Params<R> &P) {
     P.ss_lam = {(R)p->pid_ss_psi0, (R)p->pid_ss_alpha, p->pid_lambda_version};
     P.bbp_lj = {(R)p->pid_bbp_depth, (R)p->pid_bbp_rmin};
     P.bbm_lj = {(R)p->pid_bbm_depth, (R)p->pid_bbm_rmin};
+    if (ORACLE_MUTANT == 1) std::swap(P.bbp_lj, P.bbm_lj);
     R pc = 0.0;
     pc = std::max(pc, (R)(2.0 * P.bb.r_min));
     pc = std::max(pc, (R)(2.0 * P.bs.r_min));
