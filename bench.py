@@ -97,6 +97,82 @@ def valu_issue(kernel, replicas, groups, avg_ms):
     return {"issue_us": round(us, 1), "frac_of_launch": round(us / (avg_ms * 1e3), 3)}
 
 
+def host_cpu():
+    """CPU model, logical CPUs this process may use, physical cores of the
+    host (distinct (package, core) pairs)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    if phys is not None:
+        cores.add((phys, core))
+    return model, len(os.sched_getaffinity(0)), len(cores) or None
+
+
+def cpu_baseline(params, system, seconds):
+    """The CPU baseline: the oracle in the reference's OpenMP structure
+    (thread-private forces, atomic reduce, serial list rebuild and
+    integrator) built with the reference's Release flags (-O2
+    -march=native, xmd/CMakeLists.txt:32) on this host; 100 warm-up steps,
+    then the median of 5 timed runs sized to ~`seconds` in total, with every
+    thread this process may use (the OMP_NUM_THREADS share when set) and on
+    one thread (3 runs)."""
+    import subprocess
+    import tempfile
+    from oracle.oracle import Oracle
+    tmp = tempfile.mkdtemp(prefix="xmd_cpu_baseline_")
+    lib = os.path.join(tmp, "liboracle_native.so")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"OUT={lib}"], check=True)
+    model, avail, phys = host_cpu()
+    threads = avail
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+
+    def median_rate(nthreads, warm, runs, budget):
+        ora = Oracle(params, system, lib_path=lib)
+        ora.step(1, nthreads=nthreads)  # first list build (O(N^2), serial as in the reference)
+        t1 = time.perf_counter()
+        ora.step(warm, nthreads=nthreads)
+        per = (time.perf_counter() - t1) / warm
+        k = int(max(10, budget / runs / max(per, 1e-9)))
+        rates = []
+        for _ in range(runs):
+            t1 = time.perf_counter()
+            ora.step(k, nthreads=nthreads)
+            rates.append(system.n_beads * k / (time.perf_counter() - t1))
+        ora.close()
+        return float(np.median(rates)), k, [round(r, 1) for r in rates]
+
+    import shutil
+    try:
+        value, k, rates = median_rate(threads, 100, 5, seconds)
+        v1, k1, rates1 = median_rate(1, 20, 3, 0.5 * seconds)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"value": round(value, 1), "unit": "bead-steps/s", "cores": threads, "kind": "port",
+            "sample": f"one 10k-bead box (40x250, PBC), FP64 forces, reference algorithm (O(N^2) serial list "
+                      f"rebuild, per-term omp-for with thread-private forces and atomic reduce, serial Gear "
+                      f"integrator), -O2 -march=native; median of 5 runs of {k} steps after 100 warm-up steps, "
+                      f"{threads} OpenMP threads",
+            "cpu_model": model, "logical_cpus_available": avail, "host_physical_cores": phys,
+            "runs": rates,
+            "one_thread": {"value": round(v1, 1), "steps_per_run": k1, "runs": rates1}}
+
+
 def copy_calibration(mib):
     """Known-traffic kernels: a D2D copy and out = 2*x, each reading and
     writing `mib` MiB once."""
@@ -151,9 +227,22 @@ def main():
     eng.step(args.warmup)
     lanes, tune_ms = eng.autotune(reps=10)
     log(f"autotune: lanes per bead {lanes} (ms per step for 4/8/16: {[round(x, 4) for x in tune_ms]})")
+    rb0 = eng.counters()["rebuilds"]
     dt_mixed = timed_steps(eng, 0, args.steps)
     value = batch.n_beads * args.steps / dt_mixed
-    log(f"mixed batched: {args.steps} steps in {dt_mixed:.3f}s -> {value:.3e} bead-steps/s")
+    rb_window = eng.counters()["rebuilds"] - rb0
+    log(f"mixed batched: {args.steps} steps in {dt_mixed:.3f}s -> {value:.3e} bead-steps/s "
+        f"({rb_window} list rebuilds over the replicas)")
+    # the rate depends on how many list rebuilds fall in the window (one per
+    # replica every ~70 steps here): beside the K-step figure, a window of at
+    # least 2000 steps
+    long_steps = max(2000, args.steps)
+    rb1 = eng.counters()["rebuilds"]
+    dt_long = timed_steps(eng, 0, long_steps)
+    long_window = {"steps": long_steps, "value": round(batch.n_beads * long_steps / dt_long, 1),
+                   "ms_per_step": round(1e3 * dt_long / long_steps, 4),
+                   "rebuilds": eng.counters()["rebuilds"] - rb1}
+    log(f"long window: {long_window}")
     counters = eng.counters()
 
     # ---- roofline of the dominant kernel, HIP events on the engine stream
@@ -202,6 +291,7 @@ def main():
     fp64_value = None
     fp64_batched = None
     rel_rms = None
+    rel_terms = {}
     if not args.no_fp64:
         eb = Engine(p_fp64, batch)
         kb = max(100, args.steps // 10)
@@ -226,6 +316,13 @@ def main():
         Fm, _ = em.eval_forces()
         rel_rms = float(np.sqrt(((Fm - F64) ** 2).sum() / (F64 ** 2).sum()))
         log(f"mixed vs fp64 force rel-RMS: {rel_rms:.3e}")
+        for name in _abi.TERMS:
+            a, _ = e64.eval_forces(1 << _abi.TERM[name])
+            if np.abs(a).max() == 0:
+                continue
+            b, _ = em.eval_forces(1 << _abi.TERM[name])
+            rel_terms[name] = float(np.sqrt(((b - a) ** 2).sum() / (a ** 2).sum()))
+        log(f"per term: {rel_terms}")
 
     # ---- secondary workloads (mixed precision, autotuned): 512 single chains
     # of 30..400 beads in open boxes, and one 100k-bead periodic box
@@ -255,31 +352,7 @@ def main():
     # ---- CPU baseline: the oracle in the reference's OpenMP structure
     cpu = None
     if not args.no_cpu:
-        from oracle.oracle import Oracle
-        cores = min(os.cpu_count() or 1, 16)
-        ora = Oracle(p_fp64, single)
-        t1 = time.perf_counter()
-        ora.step(3, nthreads=cores)  # includes the first O(N^2) list build
-        t_first = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        ora.step(5, nthreads=cores)
-        per = (time.perf_counter() - t1) / 5
-        nsamp = int(max(10, min(5000, args.cpu_seconds / max(per, 1e-6))))
-        t1 = time.perf_counter()
-        ora.step(nsamp, nthreads=cores)
-        dtc = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        ora.step(3, nthreads=1)
-        per1 = (time.perf_counter() - t1) / 3
-        n1 = int(max(5, min(1000, 0.25 * args.cpu_seconds / max(per1, 1e-6))))
-        t1 = time.perf_counter()
-        ora.step(n1, nthreads=1)
-        dt1 = time.perf_counter() - t1
-        cpu = {"value": round(single.n_beads * nsamp / dtc, 1), "unit": "bead-steps/s", "cores": cores,
-               "kind": "port",
-               "sample": f"one 10k-bead box (40x250, PBC), FP64, {nsamp} steps after 8 warm-up steps "
-                         f"(first list build {t_first:.2f}s excluded), OpenMP {cores} threads",
-               "one_thread": {"value": round(single.n_beads * n1 / dt1, 1), "steps": n1}}
+        cpu = cpu_baseline(p_fp64, single, args.cpu_seconds)
         log(f"cpu oracle: {cpu}")
 
     out = {
@@ -305,6 +378,9 @@ def main():
         "fp64_batched": None if fp64_batched is None else round(fp64_batched, 1),
         "mixed_over_fp64_batched": None if fp64_batched is None else round(value / fp64_batched, 3),
         "force_rel_rms_mixed_vs_fp64": rel_rms,
+        "force_rel_rms_mixed_vs_fp64_per_term": rel_terms,
+        "long_window": long_window,
+        "window_rebuilds": rb_window,
         "rebuilds": counters["rebuilds"],
         "nl_entries": counters["nl_entries"],
         "qa_contacts": counters["qa_contacts"],

@@ -18,8 +18,21 @@
  *  - Vectors are SoA: x[0..n), y[0..n), z[0..n) in one buffer of 3n.
  *  - Errors: a status code plus xmd_last_error() (replaces the reference's
  *    std::runtime_error / std::out_of_range, xmd/include/xmd/store/store.h:47).
- *  - One host thread per engine; the engine owns all device memory and uses
- *    one HIP stream.
+ *  - Errors found on the device (non-finite positions, a zero uniform draw
+ *    for the noise, a list capacity exceeded) belong to one replica: that
+ *    replica is halted -- every later step leaves its state as it is --
+ *    while the other replicas go on unchanged.  The call that first finds
+ *    the error returns it, naming the replica; xmd_replica_status reports
+ *    it afterwards; xmd_set_state, xmd_set_positions or xmd_set_qa_state of
+ *    the replica clears it.  xmd_eval_forces and xmd_qa_update refuse to
+ *    run while a replica is halted.  A replica's state after the step that
+ *    failed is defined only for capacity errors of the list rebuild (the
+ *    state at the start of the next step, with incomplete lists).
+ *  - One host thread per engine; the engine owns all device memory.  It
+ *    splits its replicas into stream groups (xmd_params.stream_groups), each
+ *    with its own HIP stream and captured step graphs; xmd_step runs the
+ *    groups side by side and returns when all are done, with at most a few
+ *    graph launches (16 steps each) queued per group at any time.
  */
 #ifndef XMD_HIP_H
 #define XMD_HIP_H
@@ -312,8 +325,14 @@ xmd_status xmd_set_wall_state(xmd_engine *eng, int32_t replica, const int8_t *st
                               const double *ref_t);
 
 /* Run nsteps of the loop body (main.cpp:1028-1076 without reporters) on
- * every replica: forces, Langevin predictor-corrector, verify, rebuild. */
+ * every replica that is not halted: forces, Langevin predictor-corrector,
+ * verify, rebuild.  Returns the first error found during the call (the
+ * message names every replica that failed); replicas without errors have
+ * advanced nsteps either way. */
 xmd_status xmd_step(xmd_engine *eng, int64_t nsteps);
+/* XMD_OK, or the error that halted `replica` (xmd_last_error describes
+ * it).  No counterpart in the reference, where any error ends the run. */
+xmd_status xmd_replica_status(xmd_engine *eng, int32_t replica);
 /* Potential energy of the last force phase per replica ([n_replicas]). */
 xmd_status xmd_get_energy(xmd_engine *eng, double *V);
 /* Counters: [0] steps taken, [1] list rebuilds summed over replicas,
@@ -335,7 +354,8 @@ xmd_status xmd_rand_normals(xmd_engine *eng, uint64_t state, int64_t n,
 xmd_status xmd_profile(xmd_engine *eng, int64_t nsteps, char *names, int64_t names_cap,
                        int64_t *counts, double *total_ms, int32_t cap, int32_t *n_kernels);
 
-/* Stream used by every launch of this engine (hipStream_t as void*). */
+/* Stream of the first stream group (hipStream_t as void*); with one group
+ * (stream_groups = 1, or one replica) every launch of the engine uses it. */
 void *xmd_stream(xmd_engine *eng);
 
 /* Per-replica observables of the stats report (report_stats.cpp:8-41,

@@ -1223,10 +1223,17 @@ template <typename R> struct System : SysBase {
                 std::fill(tF.begin(), tF.end(), V3<R>{0, 0, 0});
                 R tV = 0;
                 omp_terms(s, tF, &tV);
-#pragma omp critical
-                {
-                    Vshared[0] += tV;
-                    for (int i = 0; i < s.n; ++i) s.F[i] += tF[i];
+                // reduce_vf.cpp:4-11: every thread adds its whole buffer
+                // with atomic updates (vec3_ref::atomic_add, vec3.h:93-102)
+#pragma omp atomic update
+                Vshared[0] += tV;
+                for (int i = 0; i < s.n; ++i) {
+#pragma omp atomic update
+                    s.F[i].x += tF[i].x;
+#pragma omp atomic update
+                    s.F[i].y += tF[i].y;
+#pragma omp atomic update
+                    s.F[i].z += tF[i].z;
                 }
 #pragma omp barrier
 #pragma omp master

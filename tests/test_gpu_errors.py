@@ -41,3 +41,44 @@ def test_zero_uniform_draw_reports_infinite_noise():
     with pytest.raises(XmdError) as e:
         eng.step(1)
     assert e.value.status == _abi.ERR_NONFINITE and "zero uniform draw" in str(e.value)
+
+
+# rand_gen state whose next output is 1 (multiplier inverse mod 2^64): the
+# first draw of the next step has zero upper bits, a zero uniform for log()
+_M = 0xD989BCACC137DCD5
+ZERO_NEXT = pow(_M, -1, 1 << 64)
+
+
+def test_halted_replica_leaves_the_others_unchanged():
+    system = synth.config_system(3, n_replicas=3)
+    p = synth.config_params(3, precision=_abi.PREC_FP64)
+    a, b = Engine(p, system), Engine(p, system)
+    a.step(10)
+    b.step(10)
+    y1, _, _, t1 = a.get_state(1)
+    a.set_state(1, y1, ZERO_NEXT, t1)
+    yb, _, rb, tb = b.get_state(1)
+    b.set_state(1, yb, rb, tb)  # the same forced list rebuild as in a
+    with pytest.raises(XmdError) as e:
+        a.step(5)
+    assert e.value.status == _abi.ERR_NONFINITE and "replica 1:" in str(e.value)
+    assert a.replica_status(1)[0] == _abi.ERR_NONFINITE and a.replica_status(0)[0] == 0
+    assert a.get_state(1)[3] == t1  # halted: time does not advance
+    a.step(3)  # reported once; the halted replica stays put
+    assert a.get_state(1)[3] == t1
+    with pytest.raises(XmdError) as e:
+        a.eval_forces()
+    assert e.value.status == _abi.ERR_STATE
+    b.step(8)
+    # reset replica 1 of a to b's state; b takes the same forced rebuild
+    ys, _, rs, ts = b.get_state(1)
+    a.set_state(1, ys, rs, ts)
+    b.set_state(1, ys, rs, ts)
+    assert a.replica_status(1)[0] == 0
+    a.step(7)
+    b.step(7)
+    for k in range(3):
+        ya, _, ra, ta = a.get_state(k)
+        yb, _, rb, tb = b.get_state(k)
+        assert ra == rb and ta == tb, k
+        assert np.array_equal(ya, yb), k

@@ -10,8 +10,14 @@
 * C4 (400 x 250 beads, periodic): every list bit-exact and per-term forces
   on one conformation, then 5 steps of trajectory with QA state (the
   oracle's O(N^2) list build is the cost).
+* C1 in mixed precision (FP32 pair and bonded terms): every term's forces
+  on the 100 conformations within 1e-5 relative RMS of the FP64 oracle, and
+  its energy within 1e-5 of the term's scale;
+* C4's batch of boxes at T = 0.30 + 0.2 k / 7 (8 replicas of one box at
+  C4's density and term set, 2,000 beads each so the oracle can run them),
+  each replica against the oracle running that box alone.
 Bars as in test_gpu_parity: lists equal in order, forces within 1e-10
-relative, positions within 1e-8.
+relative, positions within 1e-8, QA contacts equal field by field.
 """
 import numpy as np
 import pytest
@@ -19,7 +25,7 @@ import pytest
 from xmd_amd import synth, _abi
 from xmd_amd.engine import Engine
 from oracle.oracle import Oracle
-from test_gpu_parity import compare_terms, lists_equal
+from test_gpu_parity import compare_terms, contacts_equal, lists_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -66,9 +72,7 @@ def test_c2_full_box():
     yo, _, rngo, to = ora.get_state(0)
     assert rng == rngo and t == to
     assert np.abs(y - yo).max() < 1e-8
-    ce, se = eng.get_qa_state(0)
-    co, so = ora.get_qa_state(0)
-    assert len(ce) == len(co) and np.array_equal(se, so)
+    assert contacts_equal(eng, ora) > 0
     eng.close()
 
 
@@ -115,9 +119,65 @@ def test_c4_full_box():
     yo, _, rngo, to = ora.get_state(0)
     assert rng == rngo and t == to
     assert np.abs(y - yo).max() < 1e-8
-    ce, se = eng.get_qa_state(0)
-    co, so = ora.get_qa_state(0)
-    assert len(ce) == len(co) and np.array_equal(se, so)
+    assert contacts_equal(eng, ora) > 0
+    eng.close()
+
+
+def test_c1_mixed_hundred_conformations():
+    """C1's FP32-force protocol: per-term forces and energies of the mixed
+    engine against the FP64 oracle on the 100 seeded conformations."""
+    from test_gpu_parity import rel_rms
+    system = synth.config_system(1)
+    eng = Engine(synth.config_params(1, precision=_abi.PREC_MIXED), system)
+    ora = Oracle(synth.config_params(1), system)
+    terms = ["tether", "chiral", "nat_ang", "cnd", "pauli", "go", "const_es"]  # C1's term set
+    worst_F = {t: 0.0 for t in terms}
+    worst_V = {t: 0.0 for t in terms}
+    for seed in range(100):
+        r = _displaced(system, seed)
+        eng.set_positions(0, r)
+        ora.set_positions(0, r)
+        for t in terms:
+            m = 1 << _abi.TERM[t]
+            Fo, Vo = ora.eval_forces(m)
+            if np.abs(Fo).max() == 0:
+                continue
+            Fe, Ve = eng.eval_forces(m)
+            k = _abi.TERM[t]
+            worst_F[t] = max(worst_F[t], rel_rms(Fe, Fo))
+            # the term's scale: its energy's magnitude, at least its mean
+            # force times a bead's displacement scale over all beads
+            scale = max(abs(Vo[0, k]), 1e-3 * np.sqrt((Fo ** 2).sum()))
+            worst_V[t] = max(worst_V[t], abs(Ve[0, k] - Vo[0, k]) / scale)
+    print("C1 mixed vs FP64 oracle, worst over 100 conformations: F", worst_F, "V", worst_V, flush=True)
+    for t in terms:
+        assert worst_F[t] < 1e-5, (t, worst_F[t])
+        assert worst_V[t] < 1e-5, (t, worst_V[t])
+    eng.close()
+
+
+def test_c4_temperature_batch():
+    """C4's batched mode: boxes of C4's density and full term set at
+    T = 0.30 + 0.2 k / 7 with their own seeds, stepped as one batch; each
+    replica equals the oracle running its box alone."""
+    base = synth.config_system(4, chains=8, chain_len=250)
+    temps = [0.30 + 0.2 * k / 7 for k in range(8)]
+    seeds = [synth.SEED_BASE + 4 + 1 + k for k in range(8)]
+    batch = synth.replicate(base, temps, seeds)
+    params = synth.config_params(4)
+    print("C4 batch", batch.n_replicas, batch.n_beads, flush=True)
+    eng = Engine(params, batch)
+    eng.step(30)
+    for k in range(batch.n_replicas):
+        one = synth.replicate(base, [temps[k]], [seeds[k]])
+        ora = Oracle(params, one)
+        ora.step(30)
+        y, _, rng, t = eng.get_state(k)
+        yo, _, rngo, to = ora.get_state(0)
+        assert rng == rngo and t == to, k
+        assert np.abs(y - yo).max() < 1e-8, (k, np.abs(y - yo).max())
+        contacts_equal(eng, ora, rep=k, ora_rep=0)
+        ora.close()
     eng.close()
 
 

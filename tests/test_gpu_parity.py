@@ -145,6 +145,20 @@ def test_qa_state_bit_exact_pbc(c2_small):
         ora.step(200)
 
 
+def contacts_equal(eng, ora, rep=0, ora_rep=None):
+    """QA contact sets (i1, i2, type, status, reference time, budget
+    differences, in set order) and budgets of replica `rep` of the engine
+    and `ora_rep` (default the same) of the oracle equal; returns the
+    number of contacts."""
+    ce, se = eng.get_qa_state(rep)
+    co, so = ora.get_qa_state(rep if ora_rep is None else ora_rep)
+    assert np.array_equal(se, so), rep
+    assert len(ce) == len(co), (rep, len(ce), len(co))
+    for f in ("i1", "i2", "type", "status", "diff1", "diff2", "ref_time"):
+        assert np.array_equal(ce[f], co[f]), (rep, f)
+    return len(ce)
+
+
 def test_trajectory_fp64_pbc(c2_small):
     params, system = c2_small
     eng, ora = Engine(params, system), Oracle(params, system)
@@ -154,9 +168,7 @@ def test_trajectory_fp64_pbc(c2_small):
     yo, _, rngo, to = ora.get_state(0)
     assert rng == rngo and t == to
     assert np.abs(y - yo).max() < 1e-8
-    ce, se = eng.get_qa_state(0)
-    co, so = ora.get_qa_state(0)
-    assert len(ce) == len(co) and np.array_equal(se, so)
+    contacts_equal(eng, ora)
 
 
 def test_mixed_precision_force_error(c2_small):

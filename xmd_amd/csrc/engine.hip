@@ -155,6 +155,15 @@ struct Part {
     std::vector<void *> allocs;
     int cur = 0;
     std::string err;
+    int rep0 = 0;               // global index of the part's first replica
+    std::vector<int> reported;  // error bits of each replica already reported
+    // Graph launches queued on the stream and not yet known complete: at
+    // most MAX_INFLIGHT, the host waits for the oldest before queueing
+    // another, so one xmd_step call never holds more than that many
+    // launches (16 steps each) in flight however many steps it asks for.
+    static constexpr int MAX_INFLIGHT = 4;
+    hipEvent_t ring[MAX_INFLIGHT] = {};
+    int ring_head = 0, ring_n = 0;
     int nfp_chunks = 0, nct_chunks = 0, qa_grid = 0;
     uint32_t mask = 0;
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr}, g_one[2] = {nullptr, nullptr};
@@ -280,6 +289,8 @@ struct Part {
     }
     ~Part() {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
+        for (auto e : ring)
+            if (e) (void)hipEventDestroy(e);
         for (auto &g : g_chunk)
             if (g) (void)hipGraphExecDestroy(g);
         for (auto &g : g_one)
@@ -340,33 +351,90 @@ struct Part {
         std::fill(pending_rb.begin(), pending_rb.end(), 0);
         check_err();
     }
-    void check_err() {
-        auto e = get(D.err, 1);
-        if (e[0]) {
-            std::string m = "device error:";
-            if (e[0] & E_NL_CAP) m += " neighbour-list capacity exceeded;";
-            if (e[0] & E_FP_CAP) m += " QA free-pair capacity exceeded;";
-            if (e[0] & E_CT_CAP) m += " QA contact capacity exceeded;";
-            if (e[0] & E_SLOT) m += " more QA contacts on a bead than slots;";
-            if (e[0] & E_ZERO_DRAW)
-                throw Fail{XMD_ERR_NONFINITE, "rand_gen produced a zero uniform draw for log(): infinite Langevin "
-                                              "noise, as in the reference (rand_gen.h:45-46); positions are not finite"};
-            if (e[0] & E_NONFINITE) throw Fail{XMD_ERR_NONFINITE, "non-finite positions"};
-            throw Fail{XMD_ERR_CAPACITY, m};
+    // Error words: one per replica.  A kernel that finds an error sets its
+    // replica's word; every kernel then leaves that replica as it is
+    // (halted) until the host sets its state again (clear_err).  Each error
+    // is reported once, by the first call that finds it, naming the replica.
+    static std::string describe(int e, xmd_status &st) {
+        if (e & E_ZERO_DRAW) {
+            st = XMD_ERR_NONFINITE;
+            return "rand_gen produced a zero uniform draw for log(): infinite Langevin noise, as in the reference "
+                   "(rand_gen.h:45-46); positions are not finite";
         }
+        if (e & E_NONFINITE) {
+            st = XMD_ERR_NONFINITE;
+            return "non-finite positions";
+        }
+        st = XMD_ERR_CAPACITY;
+        std::string m = "device capacity exceeded:";
+        if (e & E_NL_CAP) m += " neighbour-list capacity exceeded;";
+        if (e & E_FP_CAP) m += " QA free-pair capacity exceeded;";
+        if (e & E_CT_CAP) m += " QA contact capacity exceeded;";
+        if (e & E_SLOT) m += " more QA contacts on a bead than slots;";
+        return m;
+    }
+    void check_err() {
+        auto e = get(D.err, nrep);
+        xmd_status first = XMD_OK;
+        std::string msg;
+        for (int k = 0; k < nrep; ++k) {
+            if (!e[k] || reported[k] == e[k]) continue;
+            reported[k] = e[k];
+            xmd_status st;
+            const std::string d = describe(e[k], st);
+            if (first == XMD_OK) first = st;
+            msg += "replica " + std::to_string(rep0 + k) + ": " + d + " (replica halted until its state is set); ";
+        }
+        if (first != XMD_OK) throw Fail{first, msg};
+    }
+    // Refuses a call whose result would cover a halted replica.
+    void require_running() {
+        check_err();
+        auto e = get(D.err, nrep);
+        for (int k = 0; k < nrep; ++k)
+            if (e[k]) {
+                xmd_status st;
+                throw Fail{XMD_ERR_STATE, "replica " + std::to_string(rep0 + k) + " is halted by an earlier error (" +
+                                              describe(e[k], st) + "); set its state first"};
+            }
+    }
+    void clear_err(int rep) {
+        const int zero = 0;
+        HIPCHK(hipMemcpy(D.err + rep, &zero, sizeof(int), hipMemcpyHostToDevice));
+        reported[rep] = 0;
+    }
+    // Waits for the oldest queued graph launch when MAX_INFLIGHT are queued.
+    void throttle() {
+        if (ring_n < MAX_INFLIGHT) return;
+        HIPCHK(hipEventSynchronize(ring[ring_head]));
+        ring_head = (ring_head + 1) % MAX_INFLIGHT;
+        --ring_n;
+    }
+    void mark_launch() {
+        hipEvent_t &ev = ring[(ring_head + ring_n) % MAX_INFLIGHT];
+        if (!ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(ev, stream));
+        ++ring_n;
+    }
+    void drain() {
+        HIPCHK(hipStreamSynchronize(stream));
+        ring_head = ring_n = 0;
     }
     // Launches the graph for the next min(n, chunk_steps) steps (no wait);
     // returns how many steps it covers.
     int64_t launch_next(int64_t n) {
+        throttle();
         if (n >= chunk_steps) {
             if (!g_chunk[cur]) g_chunk[cur] = capture(cur, chunk_steps);
             HIPCHK(hipGraphLaunch(g_chunk[cur], stream));
+            mark_launch();
             if (chunk_steps & 1) cur ^= 1;
             steps += chunk_steps;
             return chunk_steps;
         }
         if (!g_one[cur]) g_one[cur] = capture(cur, 1);
         HIPCHK(hipGraphLaunch(g_one[cur], stream));
+        mark_launch();
         cur ^= 1;
         steps += 1;
         return 1;
@@ -794,7 +862,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.rb_now = E.alloc<int>(E.nrep);
     D.force_rb = E.alloc<int>(E.nrep);
     D.any_rb = E.alloc<int>(2);
-    D.err = E.alloc<int>(1);
+    D.err = E.alloc<int>(E.nrep);
+    E.reported.assign(E.nrep, 0);
     D.n_rebuild = E.alloc<long long>(E.nrep);
     D.cell_off = E.upload(cell_off);
     D.cell_dim = E.upload(cell_dim);
@@ -1044,6 +1113,7 @@ static xmd_status part_set_state(Part *e, int32_t rep, const double *y, uint64_t
             }
         HIPCHK(hipMemcpy(e->D.rng[e->cur] + rep, &rng, sizeof(uint64_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->D.tt[e->cur] + rep, &t, sizeof(double), hipMemcpyHostToDevice));
+        e->clear_err(rep);
         e->pending_rb[rep] = 1;
     });
 }
@@ -1081,6 +1151,7 @@ static xmd_status part_set_positions(Part *e, int32_t rep, const double *r) {
         int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep];
         HIPCHK(hipStreamSynchronize(e->stream));
         put_positions(e, s0, nb, r);
+        e->clear_err(rep);
         e->pending_rb[rep] = 1;
     });
 }
@@ -1175,6 +1246,7 @@ static xmd_status part_eval_forces(Part *e, uint32_t term_mask, double *F, doubl
     if (!e || !F) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         e->ensure_lists();
+        e->require_running();
         uint32_t m = term_mask & e->mask;
         int p = e->cur;
         if (e->G.qa_on && (m & (1u << T_QA)))
@@ -1211,6 +1283,7 @@ static xmd_status part_qa_update(Part *e) {
     if (!e) return XMD_ERR_ARG;
     return run_guarded(e->err, [&]() {
         e->ensure_lists();
+        e->require_running();
         if (!e->G.qa_on) return;
         k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, e->cur, e->nfp_chunks, e->nct_chunks, 1);
         k_qa_serial<<<e->nrep, BLK, 0, e->stream>>>(e->D, e->G, e->cur);
@@ -1331,6 +1404,7 @@ static xmd_status part_set_qa_state(Part *e, int32_t rep, const xmd_qa_contact *
             HIPCHK(hipMemcpy(e->D.qslot + (size_t)q * n + s0, qslot.data() + (size_t)q * nb, nb * sizeof(int),
                              hipMemcpyHostToDevice));
         if (sync) HIPCHK(hipMemcpy(e->D.sync + s0, sync, 4 * (size_t)nb, hipMemcpyHostToDevice));
+        e->clear_err(rep);
         e->pending_rb[rep] = 1;
     });
 }
@@ -1634,6 +1708,7 @@ void create(xmd_engine &E, const xmd_params *p, const xmd_topology *t) {
         sub_topology(t, cut[g], cut[g + 1], st);
         Part *q = new Part();
         E.parts.push_back(q);
+        q->rep0 = cut[g];
         build(*q, p, &st.t, go_cut);
         E.part_rep0.push_back(cut[g]);
         E.part_bead0.push_back(t->replica_offset[cut[g]]);
@@ -1769,11 +1844,41 @@ xmd_status xmd_step(xmd_engine *e, int64_t nsteps) {
             more |= left[g] > 0;
         }
     }
+    // wait for every group, then report new errors of all groups
+    xmd_status first = XMD_OK;
+    std::string msg;
     for (Part *q : e->parts) {
-        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { q->check_err(); }));
-        if (st != XMD_OK) return st;
+        xmd_status st = run_guarded(q->err, [&]() {
+            q->drain();
+            q->check_err();
+        });
+        if (st != XMD_OK) {
+            if (first == XMD_OK) first = st;
+            msg += q->err;
+        }
     }
-    return XMD_OK;
+    if (first != XMD_OK) e->err = msg;
+    return first;
+}
+
+xmd_status xmd_replica_status(xmd_engine *e, int32_t rep) {
+    if (!e) return XMD_ERR_ARG;
+    if (rep < 0 || rep >= e->nrep) {
+        e->err = "bad replica";
+        return XMD_ERR_ARG;
+    }
+    Part *q = e->parts[e->rep_part[rep]];
+    const int lr = e->rep_local[rep];
+    return on_part(e, q, run_guarded(q->err, [&]() {
+        HIPCHK(hipStreamSynchronize(q->stream));
+        int w = 0;
+        HIPCHK(hipMemcpy(&w, q->D.err + lr, sizeof(int), hipMemcpyDeviceToHost));
+        if (w) {
+            xmd_status st;
+            const std::string d = Part::describe(w, st);
+            throw Fail{st, "replica " + std::to_string(rep) + ": " + d + " (replica halted until its state is set)"};
+        }
+    }));
 }
 
 xmd_status xmd_get_energy(xmd_engine *e, double *V) {

@@ -49,7 +49,9 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     const int s = t / LPB, sub = t % LPB;
     const int rep = D.slot_rep[s];
     const uint8_t bf = D.bflags[s];
-    const bool active = bf & B_ACTIVE;
+    // a halted replica (error word set) is skipped: nothing it computes
+    // would be used, and its wall pair states must not change
+    const bool active = (bf & B_ACTIVE) && !D.err[rep];
     const double *Pp = D.pos[p];
     const TermParams<T> &P = TT.P;
     const uint32_t mask = TT.mask;
@@ -172,7 +174,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
         }
         // neighbour row: Pauli, Debye-Hueckel, PID
         uint32_t rmask = mask & ((1u << T_PAULI) | (1u << T_CONST_ES) | (1u << T_REL_ES) | (1u << T_PID));
-        if (rmask && !(D.err[0] & E_NL_CAP)) {
+        if (rmask) {
             const int len = D.nl_len[s];
             const int base = D.slice_start[s >> 6] + (s & 63);
             const bool ces_on = mask & (1u << T_CONST_ES), res_on = mask & (1u << T_REL_ES);

@@ -99,6 +99,7 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
         const bool sift = lo < D.nrep;
         if (sift && !mutate) continue;
         const int rep = sift ? lo : lo - D.nrep;
+        if (mutate && D.err[rep]) continue;  // halted replica
         const int first = (wi - s_pre[lo]) * BLK;
         const int count = sift ? D.fp_count[rep] : D.ct_count[rep];
         const int chunk = (sift ? D.fp_base[rep] : D.ct_base[rep]) / BLK + (wi - s_pre[lo]);
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
 __device__ inline void slot_add(const Dev &D, int s, int ci) {
     int q = D.qn[s];
     if (q >= MAXQ) {
-        atomicOr(D.err, E_SLOT);
+        atomicOr(D.err + D.slot_rep[s], E_SLOT);
         return;
     }
     D.qslot[q * D.nslot + s] = ci;
@@ -237,6 +238,7 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
     __shared__ int s_n;
     __shared__ int s_state[4];  // ct count, fp count, cnt0
     int rep = blockIdx.x, tid = threadIdx.x;
+    if (D.err[rep]) return;  // halted replica: its QA state stays as it is
     int fpb = D.fp_base[rep], ctb = D.ct_base[rep];
     int fc_lo = fpb / BLK, fc_hi = fc_lo + (D.fp_count[rep] + BLK - 1) / BLK;
     double t = D.tt[p][rep];
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
                     uint32_t ns1 = sync_sub(D.sync[i1], d1), ns2 = sync_sub(D.sync[i2], d2);
                     if (!(sync_valid(ns1) && sync_valid(ns2))) continue;
                     if (cnt >= cap) {
-                        atomicOr(D.err, E_CT_CAP);
+                        atomicOr(D.err + rep, E_CT_CAP);
                         break;
                     }
                     int ci = ctb + cnt++;
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
                     D.sync[i1] = sync_add(D.sync[i1], D.ct_d1[ci]);
                     D.sync[i2] = sync_add(D.sync[i2], D.ct_d2[ci]);
                     if (fcnt >= fcap) {
-                        atomicOr(D.err, E_FP_CAP);
+                        atomicOr(D.err + rep, E_FP_CAP);
                     } else {
                         int f = fpb + fcnt++;
                         D.fp_i1[f] = i1;
@@ -407,7 +409,7 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
     float nz[3];
     bead_noise(S, D.bm_fix, nz);
     // a zero uniform in a log() slot gives infinite noise (rand_gen.h:45-46)
-    if (active && (!((rng_out(S) >> 32)) || !((rng_out(rng_step(rng_step(S))) >> 32)))) atomicOr(D.err, E_ZERO_DRAW);
+    if (active && (!((rng_out(S) >> 32)) || !((rng_out(rng_step(rng_step(S))) >> 32)))) atomicOr(D.err + rep, E_ZERO_DRAW);
     double disp = 0.0;
     if (active) {
         if (G.lang_on) {
@@ -470,6 +472,13 @@ __global__ __launch_bounds__(BLK) void k_integrate(Dev D, Derived G, int p) {
     Fy = hn ? Fy + qy : Fy;
     Fz = hn ? Fz + qz : Fz;
     if (!active) Fx = Fy = Fz = 0.0;
+    // a halted replica (error word set) keeps its state: only the
+    // positions cross to the other parity buffer (uniform per wavefront:
+    // replicas are 64-aligned)
+    if (D.err[rep]) {
+        if (active) stp(D.pos[1 - p], s, ldp(D.pos[p], s));
+        return;
+    }
     integrate_bead_impl(D, G, p, s, rep, active, Fx, Fy, Fz);
 }
 
@@ -503,6 +512,17 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
     int rep = blockIdx.x;
     int l = lane_id();
     if (rep == 0 && l == 0) D.any_rb[1 - p] = 0;
+    if (mode == 1 && D.err[rep]) {
+        // halted replica: generator and time carried over unchanged, no
+        // rebuild (a pending forced rebuild waits for the reset)
+        if (l == 0) {
+            D.rng[1 - p][rep] = D.rng[p][rep];
+            D.tt[1 - p][rep] = D.tt[p][rep];
+            D.maxd[1 - p][rep] = 0ull;
+            D.rb_now[rep] = 0;
+        }
+        return;
+    }
     if (mode == 1) {
         uint64_t Sn = wave_matvec_rows(D.jadv + 64 * rep, D.rng[p][rep]);
         if (l == 0) {
@@ -514,7 +534,7 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
         int rb = D.force_rb[rep];
         if (mode == 1) {
             double m = xmd_bm::bitsd(D.maxd[p][rep]);
-            if (!(m == m)) atomicOr(D.err, E_NONFINITE);
+            if (!(m == m)) atomicOr(D.err + rep, E_NONFINITE);
             double total = 2.0 * m + 0.0;
             if (total > G.pad) rb = 1;
             D.maxd[1 - p][rep] = 0ull;
@@ -603,7 +623,7 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
     if (tid == 0) {
         int used = s_carry - base;
         if (mode == 2) {
-            if (used > D.fp_cap[rep]) atomicOr(D.err, E_FP_CAP);
+            if (used > D.fp_cap[rep]) atomicOr(D.err + rep, E_FP_CAP);
             D.fp_count[rep] = used > D.fp_cap[rep] ? D.fp_cap[rep] : used;
         }
     }
@@ -765,7 +785,7 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
             cnt += __popcll(m);
         });
         if (cnt > W) {
-            if (l == 0) atomicOr(D.err, E_NL_CAP);
+            if (l == 0) atomicOr(D.err + rep, E_NL_CAP);
             cnt = W;
         }
     }
@@ -813,7 +833,6 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
     __shared__ int s_j[NL_WAVES][MAXSORT];
     __shared__ uint8_t s_f[NL_WAVES][MAXSORT];
     if (!D.any_rb[p]) return;
-    if (*D.err & E_NL_CAP) return;
     const int w = threadIdx.x / WAVE;
     for (int s0 = blockIdx.x * NL_WAVES; s0 < D.nslot; s0 += gridDim.x * NL_WAVES) {
         const int s = s0 + w;

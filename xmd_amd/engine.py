@@ -62,6 +62,12 @@ class Engine(HostSystem):
     def stream(self):
         return lib().xmd_stream(self.handle)
 
+    def replica_status(self, rep):
+        """(status, message) of one replica: status 0, or the error that
+        halted it (it stays halted until its state is set)."""
+        st = lib().xmd_replica_status(self.handle, C.c_int32(rep))
+        return st, (self._err() if st else "")
+
     def autotune(self, reps=10):
         """Pick the force kernel's lanes per bead (4, 8, 16) by timing;
         returns (chosen, [ms per step for 4, 8, 16])."""
