@@ -366,6 +366,17 @@ void *xmd_stream(xmd_engine *eng);
  * positions), as in the reference. */
 xmd_status xmd_observables(xmd_engine *eng, double active_threshold, double *out);
 
+/* xmd_observables without any device allocation, for callers that cannot
+ * allocate device memory (a loop under HIP graph capture, a serving loop):
+ * no counterpart in the reference.  The values are staged in a fixed device
+ * array of the library (8 doubles for each of at most
+ * XMD_OBS_NOALLOC_MAX_REPLICAS replicas; engines with more replicas get
+ * XMD_ERR_ARG), one array per GPU, shared by every call on that GPU: calls
+ * that overlap on one GPU (engines driven from two host threads) race.
+ * Every call overwrites the rows of its replicas. */
+#define XMD_OBS_NOALLOC_MAX_REPLICAS 4096
+xmd_status xmd_observables_noalloc(xmd_engine *eng, double active_threshold, double *out);
+
 /* Autotuner (no counterpart in the reference): times `reps` launches of the
  * force kernel with 4, 8 and 16 lanes per bead on the current state, keeps
  * the fastest for every later step and returns it in *lpb_out and the three

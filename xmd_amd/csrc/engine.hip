@@ -1509,6 +1509,32 @@ static xmd_status part_rand_normals(Part *e, uint64_t state, int64_t nb, float *
 
 }  // namespace
 
+// ------------------------------------------------- no-alloc staging ---
+// Staging of xmd_observables_noalloc: a file-scope device array, so that
+// call allocates nothing.  Every GPU the library runs on holds its own copy
+// until the process ends.  Host code never passes the array's name to a
+// kernel or copy (on the host it is only a stand-in): the device address
+// comes from hipGetSymbolAddress, once per device, with that device current
+// (at engine creation, the library's set-up).
+constexpr int OBS_NOALLOC_MAX_REPLICAS = XMD_OBS_NOALLOC_MAX_REPLICAS;
+static __device__ __align__(256) double g_obs_scratch[8 * OBS_NOALLOC_MAX_REPLICAS];
+namespace {
+std::mutex g_obs_mu;
+std::vector<double *> g_obs_ptr;  // device address of g_obs_scratch, by device ordinal
+double *obs_scratch_of_current_device() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_obs_mu);
+    if ((int)g_obs_ptr.size() <= dev) g_obs_ptr.resize(dev + 1, nullptr);
+    if (!g_obs_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_obs_scratch)));
+        g_obs_ptr[dev] = (double *)ptr;
+    }
+    return g_obs_ptr[dev];
+}
+}  // namespace
+
 // ================================================================ parts ===
 // An engine advances its replicas as one or more parts (contiguous replica
 // ranges of about equal bead count), each with its own device state, HIP
@@ -1522,6 +1548,7 @@ struct xmd_engine {
     std::vector<int> rep_part, rep_local, part_rep0, part_bead0;
     int nrep = 0, nbeads = 0;
     std::string err;
+    double *obs_scratch = nullptr;  // this GPU's g_obs_scratch (device address)
     ~xmd_engine() {
         for (Part *q : parts) delete q;
     }
@@ -1742,7 +1769,10 @@ xmd_status xmd_engine_create(const xmd_params *p, const xmd_topology *t, xmd_eng
     }
     *out = nullptr;
     xmd_engine *e = new xmd_engine();
-    xmd_status st = run_guarded(g_create_err, [&]() { create(*e, p, t); });
+    xmd_status st = run_guarded(g_create_err, [&]() {
+        create(*e, p, t);
+        e->obs_scratch = obs_scratch_of_current_device();
+    });
     if (st != XMD_OK) {
         delete e;
         return st;
@@ -1960,6 +1990,36 @@ xmd_status xmd_observables(xmd_engine *e, double active_threshold, double *out) 
             HIPCHK(hipGetLastError());
             auto h = q->get(d.p, 8 * (size_t)q->nrep);
             std::memcpy(out + 8 * (size_t)e->part_rep0[g], h.data(), h.size() * sizeof(double));
+        }));
+        if (st != XMD_OK) return st;
+    }
+    return XMD_OK;
+}
+
+xmd_status xmd_observables_noalloc(xmd_engine *e, double active_threshold, double *out) {
+    if (!e || !out) return XMD_ERR_ARG;
+    if (e->nrep > OBS_NOALLOC_MAX_REPLICAS) {
+        e->err = "xmd_observables_noalloc: " + std::to_string(e->nrep) + " replicas, at most " +
+                 std::to_string(OBS_NOALLOC_MAX_REPLICAS);
+        return XMD_ERR_ARG;
+    }
+    // each stream group writes its own replicas' rows of the staging array
+    for (size_t g = 0; g < e->parts.size(); ++g) {
+        Part *q = e->parts[g];
+        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() {
+            k_observables<<<q->nrep, BLK, 0, q->stream>>>(q->D, q->G, q->cur, active_threshold,
+                                                          e->obs_scratch + 8 * (size_t)e->part_rep0[g]);
+            HIPCHK(hipGetLastError());
+        }));
+        if (st != XMD_OK) return st;
+    }
+    for (size_t g = 0; g < e->parts.size(); ++g) {
+        Part *q = e->parts[g];
+        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() {
+            const size_t off = 8 * (size_t)e->part_rep0[g];
+            HIPCHK(hipMemcpyAsync(out + off, e->obs_scratch + off, 8 * (size_t)q->nrep * sizeof(double),
+                                  hipMemcpyDeviceToHost, q->stream));
+            HIPCHK(hipStreamSynchronize(q->stream));
         }));
         if (st != XMD_OK) return st;
     }

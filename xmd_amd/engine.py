@@ -86,6 +86,15 @@ class Engine(HostSystem):
                                           out.ctypes.data_as(C.POINTER(C.c_double))))
         return out.reshape(self.nrep, 8)
 
+    def observables_noalloc(self, active_threshold=1.5):
+        """observables() through the entry point that allocates no device
+        memory (a fixed staging array of the library, at most 4096
+        replicas)."""
+        out = np.zeros(8 * self.nrep)
+        self._check(lib().xmd_observables_noalloc(self.handle, C.c_double(active_threshold),
+                                                  out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(self.nrep, 8)
+
     def rand_normals(self, state, n):
         out = np.zeros(3 * max(n, 1), dtype=np.float32)
         so = C.c_uint64()
