@@ -8,12 +8,12 @@
 namespace xmd_k {
 
 // Force on every slot from every enabled term.  LPB consecutive lanes share
-// a bead: the 8 bonded tasks (4 quads, 3 angles, the two tethers; native or
-// heuristic angles and dihedrals by whether a native value exists) and the
-// neighbour-row entries are strided over the lanes, then a shuffle tree
-// (fixed order, hence deterministic) sums the lanes' partial forces.  The
-// result goes to Fbuf (plus the PID spill onto chain neighbours); the
-// integrator runs in k_integrate.
+// a bead: its bonded contributions (computed once per quad, angle and bond
+// by the block, bonded_phase), native contacts and neighbour-row entries are
+// strided over the lanes, then a shuffle tree (fixed order, hence
+// deterministic) sums the lanes' partial forces.  The result goes to Fbuf
+// (plus the PID spill onto chain neighbours); the integrator runs in
+// k_integrate.
 template <typename T>
 __device__ inline void add3(double &x, double &y, double &z, v3<T> f) {
     x += (double)f.x;
@@ -41,12 +41,125 @@ template <int LPB> __device__ inline double lane_sum(double v) {
     return v;
 }
 
+// Bonded terms of one block of beads [s0, s0 + NB), each quad, angle and
+// bond computed once: quads starting at s0-3 .. s0+NB-1, angles and bonds
+// starting at s0-2 .. s0+NB-1 (the few starting left of the block are
+// computed again by the block on the left: the same inputs give the same
+// bits).  Forces per bead position and energies per term, in FP64.
+constexpr int QUAD_TERMS[4] = {T_CHIRAL, T_CND, T_SND, T_HEUR_DIH};
+constexpr int ANG_TERMS[2] = {T_NAT_ANG, T_HEUR_ANG};
+template <int LPB> struct BondedLds {
+    static constexpr int NB = BLK / LPB, NQ = NB + 3, NA = NB + 2;
+    v3<double> qF[NQ][4];
+    double qV[NQ][4];
+    v3<double> aF[NA][3];
+    double aV[NA][2];
+    v3<double> tF[NA][2];
+    double tV[NA];
+};
+
+// Quad of beads first..first+3: chirality, native dihedrals, heuristic
+// dihedral (xmd/src/forces/chiral.cpp:15-37, dihedral/complex_native.cpp:
+// 13-49, simple_native.cpp:13-45, dihedral/heurestic.cpp:29-70).
+template <typename T, int LPB>
+__device__ inline void quad_task(const Dev &D, const TermParams<T> &P, uint32_t mask, const double *Pp, int first,
+                                 BondedLds<LPB> &L, int q) {
+    v3<double> F[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double V[4] = {0, 0, 0, 0};
+    const int n = D.nslot;
+    const bool ok = first >= 0 && first + 3 < n && D.chain[first] >= 0 && D.chain[first] == D.chain[first + 3] &&
+                    !D.err[D.slot_rep[first]];
+    if (ok && (mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND) | (1u << T_HEUR_DIH)))) {
+        const v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2),
+                         q3 = ldp(Pp, first + 3);
+        const DihFrame<T> fr = dih_frame<T>(q1 - q0, q2 - q1, q3 - q2);
+        v3<T> out[4];
+        auto acc = [&](int term, T v) {
+            for (int k = 0; k < 4; ++k) F[k] = F[k] + cvt<double>(out[k]);
+            V[term] += (double)v;
+        };
+        if (mask & (1u << T_CHIRAL)) {
+            const double nc = D.chir_nat[first];
+            if (nc == nc) acc(0, chiral<T>(fr.r12, fr.r23, fr.r34, P.e_chi, (T)nc, (T)D.chir_fac[first], out));
+        }
+        const double ph = D.phi_nat[first];
+        if (ph == ph) {
+            if (mask & (1u << T_CND)) acc(1, nat_dihedral<T>(fr, 0, P.CDA, P.CDB, P.CDH, (T)ph, out));
+            if (mask & (1u << T_SND)) acc(2, nat_dihedral<T>(fr, 1, P.CDA, P.CDB, P.CDH, (T)ph, out));
+        } else if ((mask & (1u << T_HEUR_DIH)) && (D.bflags[first] & B_HDIH)) {
+            // typed by the residues of beads 2 and 3
+            const double *cd = D.heur_dih_c + 6 * (3 * heur_class(D.atype[first + 1]) + heur_class(D.atype[first + 2]));
+            T c[6];
+            for (int k = 0; k < 6; ++k) c[k] = (T)cd[k];
+            acc(3, heur_dihedral<T>(fr, c, out));
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        L.qF[q][k] = F[k];
+        L.qV[q][k] = V[k];
+    }
+}
+
+// Angle of beads first..first+2 (native, else heuristic; angle/native.cpp:
+// 13-39, angle/heurestic.cpp:29-59) and the bond (first, first+1)
+// (tether.cpp:21-36, FP64 in both forms: stiff bonds cancel in r - r_nat).
+template <typename T, int LPB>
+__device__ inline void angle_bond_task(const Dev &D, const Derived &G, const TermParams<T> &P, uint32_t mask,
+                                       const double *Pp, int first, BondedLds<LPB> &L, int a) {
+    v3<double> F[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, B[2] = {{0, 0, 0}, {0, 0, 0}};
+    double V[2] = {0, 0}, Vb = 0;
+    const int n = D.nslot;
+    const int ch = first >= 0 && first < n ? D.chain[first] : -1;
+    const bool live = ch >= 0 && !D.err[D.slot_rep[first]];
+    if (live && first + 2 < n && D.chain[first + 2] == ch && (mask & ((1u << T_NAT_ANG) | (1u << T_HEUR_ANG)))) {
+        const v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
+        const double th = D.theta_nat[first];
+        if (th == th) {
+            if (mask & (1u << T_NAT_ANG)) {
+                v3<T> out[3];
+                V[0] = (double)nat_angle<T>(q1 - q0, q2 - q1, P.ang_k, (T)th, out);
+                for (int k = 0; k < 3; ++k) F[k] = cvt<double>(out[k]);
+            }
+        } else if ((mask & (1u << T_HEUR_ANG)) && (D.bflags[first] & B_HANG)) {
+            // typed by the residues of beads 1 and 2, FP64 in both forms
+            const double *ca = D.heur_ang_c + 7 * (3 * heur_class(D.atype[first]) + heur_class(D.atype[first + 1]));
+            V[1] = heur_angle(q1 - q0, q2 - q1, ca, F);
+        }
+    }
+    if (live && first + 1 < n && D.chain[first + 1] == ch && (mask & (1u << T_TETHER)))
+        Vb = tether<double>(ldp(Pp, first + 1) - ldp(Pp, first), G.H1, G.H2, D.tether_nat[first], B);
+    for (int k = 0; k < 3; ++k) L.aF[a][k] = F[k];
+    L.aV[a][0] = V[0];
+    L.aV[a][1] = V[1];
+    L.tF[a][0] = B[0];
+    L.tF[a][1] = B[1];
+    L.tV[a] = Vb;
+}
+
+// Whole wavefronts per task kind (quads from wave 0, angles and bonds from
+// the next free wave) so that no wavefront runs both code paths.
+template <typename T, int LPB>
+__device__ inline void bonded_phase(const Dev &D, const Derived &G, const TermParams<T> &P, uint32_t mask,
+                                    const double *Pp, int s0, BondedLds<LPB> &L) {
+    using BL = BondedLds<LPB>;
+    constexpr int A0 = (BL::NQ + WAVE - 1) / WAVE * WAVE;
+    for (int task = threadIdx.x; task < A0 + BL::NA; task += BLK) {
+        if (task < BL::NQ) quad_task<T, LPB>(D, P, mask, Pp, s0 - 3 + task, L, task);
+        else if (task >= A0) angle_bond_task<T, LPB>(D, G, P, mask, Pp, s0 - 2 + (task - A0), L, task - A0);
+    }
+    __syncthreads();
+}
+
 template <typename T, bool EVAL, int LPB>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     const int n = D.nslot;
-    const int t = xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int t = blk * BLK + threadIdx.x;
     const int s = t / LPB, sub = t % LPB;
+    const int s0 = blk * (BLK / LPB);  // the block's first bead
+    __shared__ BondedLds<LPB> L;
+    bonded_phase<T, LPB>(D, G, TT.P, TT.mask, D.pos[p], s0, L);
     const int rep = D.slot_rep[s];
     const uint8_t bf = D.bflags[s];
     // a halted replica (error word set) is skipped: nothing it computes
@@ -66,87 +179,27 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     if (active) {
         const v3<double> ri = ldp(Pp, s);
         const Box box = D.box[rep];
-        const int ch = D.chain[s];
-        auto in_chain = [&](int a) { return a >= 0 && a < n && D.chain[a] == ch; };
-        // bonded tasks: 0..3 quad starting at s-3+task, 4..6 angle starting
-        // at s-2+(task-4), 7 tethers
-        for (int task = sub; task < 8; task += LPB) {
-            if (task < 4) {
-                int a = task, first = s - 3 + a;
-                if (!(mask & ((1u << T_CHIRAL) | (1u << T_CND) | (1u << T_SND) | (1u << T_HEUR_DIH)))) continue;
-                if (!in_chain(first) || !in_chain(first + 3)) continue;
-                v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2),
-                           q3 = ldp(Pp, first + 3);
-                const DihFrame<T> fr = dih_frame<T>(q1 - q0, q2 - q1, q3 - q2);
-                const v3<T> r12 = fr.r12, r23 = fr.r23, r34 = fr.r34;
-                v3<T> out[4];
-                if (mask & (1u << T_CHIRAL)) {
-                    double nc = D.chir_nat[first];
-                    if (nc == nc) {
-                        T v = chiral<T>(r12, r23, r34, P.e_chi, (T)nc, (T)D.chir_fac[first], out);
-                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                        if (a == 3) addV(T_CHIRAL, (double)v);
-                    }
-                }
-                double ph = D.phi_nat[first];
-                if (ph == ph) {
-                    if (mask & (1u << T_CND)) {
-                        T v = nat_dihedral<T>(fr, 0, P.CDA, P.CDB, P.CDH, (T)ph, out);
-                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                        if (a == 3) addV(T_CND, (double)v);
-                    }
-                    if (mask & (1u << T_SND)) {
-                        T v = nat_dihedral<T>(fr, 1, P.CDA, P.CDB, P.CDH, (T)ph, out);
-                        add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                        if (a == 3) addV(T_SND, (double)v);
-                    }
-                } else if ((mask & (1u << T_HEUR_DIH)) && (D.bflags[first] & B_HDIH)) {
-                    // typed by the residues of beads 2 and 3
-                    const double *cd =
-                        D.heur_dih_c + 6 * (3 * heur_class(D.atype[first + 1]) + heur_class(D.atype[first + 2]));
-                    T c[6];
-                    for (int k = 0; k < 6; ++k) c[k] = (T)cd[k];
-                    T v = heur_dihedral<T>(fr, c, out);
-                    add3(Fx, Fy, Fz, pick4(out, 3 - a));
-                    if (a == 3) addV(T_HEUR_DIH, (double)v);
-                }
-            } else if (task < 7) {
-                int a = task - 4, first = s - 2 + a;
-                if (!(mask & ((1u << T_NAT_ANG) | (1u << T_HEUR_ANG)))) continue;
-                if (!in_chain(first) || !in_chain(first + 2)) continue;
-                double th = D.theta_nat[first];
-                if (th == th) {
-                    if (!(mask & (1u << T_NAT_ANG))) continue;
-                    v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
-                    v3<T> out[3];
-                    T v = nat_angle<T>(q1 - q0, q2 - q1, P.ang_k, (T)th, out);
-                    add3(Fx, Fy, Fz, pick3(out, 2 - a));
-                    if (a == 2) addV(T_NAT_ANG, (double)v);
-                } else {
-                    // no native value: the heuristic angle, typed by the
-                    // residues of beads 1 and 2, FP64 in both forms
-                    if (!(mask & (1u << T_HEUR_ANG)) || !(D.bflags[first] & B_HANG)) continue;
-                    const double *ca =
-                        D.heur_ang_c + 7 * (3 * heur_class(D.atype[first]) + heur_class(D.atype[first + 1]));
-                    v3<double> q0 = ldp(Pp, first), q1 = ldp(Pp, first + 1), q2 = ldp(Pp, first + 2);
-                    v3<double> out[3];
-                    double v = heur_angle(q1 - q0, q2 - q1, ca, out);
-                    add3(Fx, Fy, Fz, pick3(out, 2 - a));
-                    if (a == 2) addV(T_HEUR_ANG, v);
-                }
+        // bonded terms: the block computed each quad, angle and tether once
+        // (bonded_phase); the bead sums the ones it belongs to, in a fixed
+        // order: quads starting at s-3..s, angles at s-2..s, bonds (s-1, s)
+        // and (s, s+1); an energy is the first bead's
+        const int lo = s - s0;
+        for (int c = sub; c < 9; c += LPB) {
+            if (c < 4) {
+                const int q = lo + c;  // quad starting at s - 3 + c, this bead at position 3 - c
+                add3(Fx, Fy, Fz, L.qF[q][3 - c]);
+                if (c == 3)
+                    for (int k = 0; k < 4; ++k) addV(QUAD_TERMS[k], L.qV[q][k]);
+            } else if (c < 7) {
+                const int a = lo + (c - 4);  // angle starting at s - 2 + (c - 4)
+                add3(Fx, Fy, Fz, L.aF[a][6 - c]);
+                if (c == 6)
+                    for (int k = 0; k < 2; ++k) addV(ANG_TERMS[k], L.aV[a][k]);
+            } else if (c == 7) {
+                add3(Fx, Fy, Fz, L.tF[lo + 1][1]);  // bond (s - 1, s)
             } else {
-                // FP64 in both forms: stiff bonds cancel in r - r_nat
-                if (!(mask & (1u << T_TETHER))) continue;
-                v3<double> out[2];
-                if (bf & B_PREV) {
-                    tether<double>(ri - ldp(Pp, s - 1), G.H1, G.H2, D.tether_nat[s - 1], out);
-                    add3(Fx, Fy, Fz, out[1]);
-                }
-                if (bf & B_NEXT) {
-                    double v = tether<double>(ldp(Pp, s + 1) - ri, G.H1, G.H2, D.tether_nat[s], out);
-                    add3(Fx, Fy, Fz, out[0]);
-                    addV(T_TETHER, v);
-                }
+                add3(Fx, Fy, Fz, L.tF[lo + 2][0]);  // bond (s, s + 1)
+                addV(T_TETHER, L.tV[lo + 2]);
             }
         }
         // native contacts (Go)
