@@ -72,3 +72,48 @@ def test_mixed_precision_samples_the_fp64_ensemble():
         assert abs(d.mean()) < 4 * se, (name, a[:, :, q].mean(), b[:, :, q].mean(), se)
     emx.close()
     e64.close()
+
+
+def host_sample(hosts, systems, nsteps, every, burn):
+    """Per-sample, per-replica (Rg, K, native fraction, QA contacts) of a
+    list of drivers (engine or oracles, one system each), stepped side by
+    side on threads (the oracle's calls release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(hosts)) as ex:
+        list(ex.map(lambda h: h.step(burn), hosts))
+        out = []
+        for _ in range(nsteps // every):
+            list(ex.map(lambda h: h.step(every), hosts))
+            row = []
+            for h, sy in zip(hosts, systems):
+                for k in range(sy.n_replicas):
+                    row.append(list(replica_observables(h, sy, k)) + [len(h.get_qa_state(k)[0])])
+            out.append(row)
+    return np.array(out)  # (samples, replicas, 4)
+
+
+def test_full_term_set_ensemble_matches_oracle():
+    """The FP64 engine against the oracle on the full term set (Go, Pauli,
+    Debye-Hueckel, PID, QA, tether, chirality, native angles and
+    dihedrals): 32 boxes of 4 x 60 beads at C2's density and T = 0.30 ..
+    0.50, the same initial states and noise streams; trajectories
+    decorrelate, so per-replica time averages of Rg, K, native fraction and
+    QA contacts are compared as paired differences within 4 standard
+    errors (reference: stats/gyration.cpp:4-18, report_structure.cpp:81-85,
+    dynamics/lang_pc.cpp:5-41)."""
+    nrep, ngroups = 32, 8
+    system = synth.config_system(2, n_batch=nrep, chains=4, chain_len=60, horizon_steps=6000)
+    params = synth.config_params(2)
+    eng = Engine(params, system)
+    groups = [list(range(g * nrep // ngroups, (g + 1) * nrep // ngroups)) for g in range(ngroups)]
+    subs = [synth.subset(system, ks) for ks in groups]
+    oras = [Oracle(params, sy) for sy in subs]
+    a = host_sample([eng], [system], 4000, 200, 1000)
+    b = host_sample(oras, subs, 4000, 200, 1000)
+    assert b[..., 3].mean() > 1  # QA contacts form
+    for q, name in enumerate(["Rg", "K", "Q", "QA contacts"]):
+        d = a[:, :, q].mean(axis=0) - b[:, :, q].mean(axis=0)
+        se = d.std(ddof=1) / np.sqrt(nrep) + 1e-12 * abs(b[:, :, q].mean())
+        print(name, a[:, :, q].mean(), b[:, :, q].mean(), d.mean(), se, flush=True)
+        assert abs(d.mean()) < 4 * se, (name, a[:, :, q].mean(), b[:, :, q].mean(), se)
+    eng.close()

@@ -29,21 +29,33 @@ BOND = 3.8
 MIN_SEP = 4.5
 NAT_CUT = 7.5
 SEED_BASE = 0x5EED0000
+# Largest bond angle of the configurations' native chains (degrees).  A
+# self-avoiding walk with uniform bond directions puts ~5 native angles per
+# 10k beads within 2 degrees of a straight line, where the native dihedral
+# and PID gradients (1/|r12 x r23|) diverge and the reference's force field
+# blows up within a few thousand steps; C-alpha traces of proteins have bond
+# angles of about 80-150 degrees.
+MAX_BOND_ANGLE = 150.0
 
 
-def saw_chain(n, rng, bond=BOND, min_sep=MIN_SEP, max_tries=200):
+def saw_chain(n, rng, bond=BOND, min_sep=MIN_SEP, max_tries=200, max_angle=None):
     """Self-avoiding walk of n beads (A), grown bead by bead with
-    back-tracking when a bead cannot be placed."""
+    back-tracking when a bead cannot be placed; with max_angle (degrees)
+    every bond angle (at bead k between beads k-1 and k+1) is at most that."""
     pos = np.zeros((n, 3))
     k = 1
     fails = np.zeros(n, dtype=np.int64)
     min2 = min_sep * min_sep
+    # bond angle <= max_angle  <=>  b_prev . b_next <= -cos(max_angle) bond^2
+    max_dot = None if max_angle is None else -np.cos(np.radians(max_angle)) * bond * bond
     while k < n:
         placed = False
         for _ in range(max_tries):
             d = rng.normal(size=3)
             d *= bond / np.linalg.norm(d)
             cand = pos[k - 1] + d
+            if max_dot is not None and k >= 2 and np.dot(pos[k - 1] - pos[k - 2], d) > max_dot:
+                continue
             if k >= 2:
                 diff = pos[:k - 1] - cand
                 if np.min(np.einsum("ij,ij->i", diff, diff)) < min2:
@@ -271,8 +283,8 @@ def make_system(replicas):
     )
 
 
-def random_chain(n, rng):
-    return saw_chain(n, rng), rng.integers(0, 20, size=n)
+def random_chain(n, rng, max_angle=None):
+    return saw_chain(n, rng, max_angle=max_angle), rng.integers(0, 20, size=n)
 
 
 # Term sets of the configurations (names of the YAML sections' enabled flags).
@@ -325,7 +337,8 @@ def _pick_seeds(start, count, n_beads_of, horizon):
     return seeds
 
 
-def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, seed=None, horizon_steps=0):
+def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, seed=None, horizon_steps=0,
+                  max_angle=MAX_BOND_ANGLE):
     """Synthetic system of configuration cid.
 
     C0: 1 x 60, open box.  C1: 1 x 1000, open box.
@@ -335,7 +348,8 @@ def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, 
     n_batch > 1 repeats C2/C4 as independent boxes with temperatures
     0.30 + 0.2 k/(n_batch-1) and seeds base + k.  With horizon_steps > 0,
     seeds whose noise stream hits a zero log() draw within that many steps
-    are skipped (the next seed is taken).
+    are skipped (the next seed is taken).  Native chains have bond angles of
+    at most max_angle degrees (None: any).
     """
     base = SEED_BASE + cid if seed is None else seed
     reps = []
@@ -343,7 +357,7 @@ def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, 
         n = {0: 60, 1: 1000}[cid] if chain_len is None else chain_len
         for k, sd in enumerate(_pick_seeds(base, n_batch, lambda _: n, horizon_steps)):
             rng = np.random.default_rng(sd)
-            reps.append(dict(chains=[random_chain(n, rng)], box_A=(0, 0, 0), temperature=0.35, seed=sd))
+            reps.append(dict(chains=[random_chain(n, rng, max_angle)], box_A=(0, 0, 0), temperature=0.35, seed=sd))
     elif cid in (2, 4):
         nch = {2: 40, 4: 400}[cid] if chains is None else chains
         m = 250 if chain_len is None else chain_len
@@ -351,7 +365,7 @@ def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, 
         L = (nch * m / dens) ** (1.0 / 3.0)
         for k, sd in enumerate(_pick_seeds(base, n_batch, lambda _: nch * m, horizon_steps)):
             rng = np.random.default_rng(sd)
-            chs = [random_chain(m, rng) for _ in range(nch)]
+            chs = [random_chain(m, rng, max_angle) for _ in range(nch)]
             placed = place_chains([c[0] for c in chs], L, rng)
             T = 0.35 if n_batch == 1 else 0.30 + 0.2 * k / (n_batch - 1)
             reps.append(dict(chains=[(p, c[1]) for p, c in zip(placed, chs)], box_A=(L, L, L),
@@ -363,7 +377,7 @@ def config_system(cid, n_batch=1, n_replicas=None, chains=None, chain_len=None, 
         temps = rng0.uniform(0.30, 0.50, size=nrep)
         for k, sd in enumerate(_pick_seeds(base + 1, nrep, lambda i: int(lens[i]), horizon_steps)):
             rng = np.random.default_rng(sd)
-            reps.append(dict(chains=[random_chain(int(lens[k]), rng)], box_A=(0, 0, 0),
+            reps.append(dict(chains=[random_chain(int(lens[k]), rng, max_angle)], box_A=(0, 0, 0),
                              temperature=float(temps[k]), seed=sd))
     else:
         raise ValueError(cid)
@@ -425,6 +439,30 @@ def concat(systems):
         force_afm_F=np.concatenate([s.force_afm_F.reshape(-1, 3) for s in systems]),
         heur_bonded=None if all(s.heur_bonded is None for s in systems) else np.concatenate(
             [np.zeros(s.n_beads, np.uint8) if s.heur_bonded is None else s.heur_bonded for s in systems]))
+
+
+def subset(system, replicas):
+    """The replicas `replicas` (in that order) of a batch as a batch of
+    their own."""
+    parts = []
+    for k in replicas:
+        sl = system.replica_slice(k)
+        b0, n = sl.start, sl.stop - sl.start
+        m = (system.nat_i1 >= b0) & (system.nat_i1 < sl.stop)
+        ms = (system.ss_i1 >= b0) & (system.ss_i1 < sl.stop)
+        co = system.chain_offset
+        ch = co[(co >= b0) & (co <= sl.stop)] - b0
+        hb = None if system.heur_bonded is None else system.heur_bonded[sl]
+        parts.append(System(
+            replica_offset=np.array([0, n], dtype=np.int32), chain_offset=ch.astype(np.int32),
+            atype=system.atype[sl], r0=system.r0[:, sl], box=system.box.reshape(-1, 3)[k:k + 1],
+            temperature=system.temperature[k:k + 1], rng_state=system.rng_state[k:k + 1],
+            nat_i1=(system.nat_i1[m] - b0).astype(np.int32), nat_i2=(system.nat_i2[m] - b0).astype(np.int32),
+            nat_dist=system.nat_dist[m], tether_len=system.tether_len[sl], nat_theta=system.nat_theta[sl],
+            nat_phi=system.nat_phi[sl], nat_chir=system.nat_chir[sl], nat_chir_factor=system.nat_chir_factor[sl],
+            ss_i1=(system.ss_i1[ms] - b0).astype(np.int32), ss_i2=(system.ss_i2[ms] - b0).astype(np.int32),
+            solid_walls=system.solid_walls, lj_walls=system.lj_walls, heur_bonded=hb))
+    return concat(parts)
 
 
 def perturbed_positions(system, seed, sigma_A=0.3):
