@@ -200,6 +200,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 512-chain and 100k-bead workloads")
     ap.add_argument("--no-step-roofline", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=300)
+    ap.add_argument("--long-steps", type=int, default=2000,
+                    help="steps of the second timed window (0: none); the rate depends on how many list "
+                         "rebuilds fall in a window, so it is reported beside the --steps one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy and a scale kernel of MIB MiB each "
                          "(known HBM bytes, to calibrate FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -236,13 +239,15 @@ def main():
     # the rate depends on how many list rebuilds fall in the window (one per
     # replica every ~70 steps here): beside the K-step figure, a window of at
     # least 2000 steps
-    long_steps = max(2000, args.steps)
-    rb1 = eng.counters()["rebuilds"]
-    dt_long = timed_steps(eng, 0, long_steps)
-    long_window = {"steps": long_steps, "value": round(batch.n_beads * long_steps / dt_long, 1),
-                   "ms_per_step": round(1e3 * dt_long / long_steps, 4),
-                   "rebuilds": eng.counters()["rebuilds"] - rb1}
-    log(f"long window: {long_window}")
+    long_window = None
+    if args.long_steps > 0:
+        long_steps = max(args.long_steps, args.steps)
+        rb1 = eng.counters()["rebuilds"]
+        dt_long = timed_steps(eng, 0, long_steps)
+        long_window = {"steps": long_steps, "value": round(batch.n_beads * long_steps / dt_long, 1),
+                       "ms_per_step": round(1e3 * dt_long / long_steps, 4),
+                       "rebuilds": eng.counters()["rebuilds"] - rb1}
+        log(f"long window: {long_window}")
     counters = eng.counters()
 
     # ---- roofline of the dominant kernel, HIP events on the engine stream

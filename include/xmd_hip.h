@@ -194,6 +194,14 @@ typedef struct xmd_params {
     int32_t heur_ang_enabled, heur_dih_enabled;
     double heur_ang_coeffs[9][7];
     double heur_dih_coeffs[9][6];
+    /* width of the neighbour rows (entries per bead): 0 = automatic (three
+       times the longest row of the first list plus 64, at least 256), n > 0
+       = n but at least the longest first row; at most 1024.  A rebuild that
+       needs wider rows widens them and steps the replica that needed them
+       alone back to the others (the same results as an engine created with
+       the wider rows).  No counterpart in freeform_params, whose lists are
+       vectors that grow (nl/legacy_update.cpp:28-59). */
+    int32_t row_width;
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and

@@ -1,0 +1,37 @@
+"""Neighbour rows that outgrow their width (the reference's lists are
+vectors that grow, xmd/src/nl/legacy_update.cpp:28-59): the engine widens
+the rows, rebuilds the replica that needed it from the state it halted at
+and steps it alone back to the others.  Every kernel exits normally: the
+overflow is a flag the row fill sets and the host reads after the step."""
+import numpy as np
+import pytest
+
+from xmd_amd import synth
+from xmd_amd.engine import Engine
+from test_gpu_parity import contacts_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rows_grow_like_a_wider_engine():
+    """Rows exactly as wide as the longest first row (row_width 1) against
+    rows of 1024 entries: after 3000 steps every replica's state, QA state
+    and the rebuild count are equal bit for bit, and the rows have grown."""
+    system = synth.config_system(2, n_batch=3, chains=8, chain_len=60, horizon_steps=4000)
+    tight, wide = synth.config_params(2), synth.config_params(2)
+    tight.row_width, wide.row_width = 1, 1024
+    a, b = Engine(tight, system), Engine(wide, system)
+    cap0 = a.counters()["nl_capacity"]
+    for _ in range(6):
+        a.step(500)
+        b.step(500)
+    ca, cb = a.counters(), b.counters()
+    print("capacity", cap0, "->", ca["nl_capacity"], "of", cb["nl_capacity"], flush=True)
+    assert ca["nl_capacity"] > cap0
+    assert ca["steps"] == cb["steps"] == 3000 and ca["rebuilds"] == cb["rebuilds"]
+    for k in range(system.n_replicas):
+        ya, _, ra, ta = a.get_state(k)
+        yb, _, rb, tb = b.get_state(k)
+        assert ra == rb and ta == tb, k
+        assert np.array_equal(ya, yb), k
+        contacts_equal(a, b, rep=k)

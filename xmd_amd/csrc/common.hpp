@@ -20,7 +20,9 @@ enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16, 
 // B_HANG / B_HDIH: the model's angle (dihedral) starting at this bead has no
 // native value, so the heuristic term takes it when enabled.
 enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4, B_HANG = 8, B_HDIH = 16 };
-enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32 };
+// E_HOLD is no error: the host holds the replica (every kernel leaves it as
+// it is, like a halted one) while it steps another alone.
+enum : int { E_NL_CAP = 1, E_FP_CAP = 2, E_CT_CAP = 4, E_SLOT = 8, E_NONFINITE = 16, E_ZERO_DRAW = 32, E_HOLD = 64 };
 enum : int {
     T_TETHER = 0, T_CHIRAL, T_NAT_ANG, T_CND, T_SND, T_PAULI, T_GO, T_CONST_ES, T_REL_ES, T_QA, T_PID, T_NAT_SS,
     T_SOLID_WALL, T_LJ_WALL, T_VEL_AFM, T_FORCE_AFM, T_HEUR_ANG, T_HEUR_DIH
@@ -57,6 +59,8 @@ struct Dev {
     unsigned long long *maxd[2];
     int *rb_now, *force_rb, *any_rb, *err;
     long long *n_rebuild;
+    long long *n_done;  // steps each replica has taken (halted ones stop counting)
+    int *need_w;        // longest row a rebuild of the replica wanted beyond the row width
     const int *cell_off, *cell_dim;  // [nrep+1], [3*nrep]
     int *cell_cnt, *cell_start, *cell_end, *cell_cursor, *cell_beads, *bead_cell;
     // slots

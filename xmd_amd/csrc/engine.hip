@@ -193,20 +193,10 @@ struct Part {
     // Use `l` lanes per bead from now on (graphs are captured again).
     void set_lpb(int l) {
         if (l == lpb) return;
-        HIPCHK(hipStreamSynchronize(stream));
-        for (auto &g : g_chunk)
-            if (g) {
-                HIPCHK(hipGraphExecDestroy(g));
-                g = nullptr;
-            }
-        for (auto &g : g_one)
-            if (g) {
-                HIPCHK(hipGraphExecDestroy(g));
-                g = nullptr;
-            }
+        drop_graphs();
         lpb = l;
     }
-    int64_t steps = 0;
+    int64_t steps = 0, catchup_steps = 0;  // graph steps launched; those spent catching a replica up
     std::vector<char> pending_rb;
     std::vector<int> fp_base_h, fp_cap_h, ct_base_h, ct_cap_h;
     long long nl_total = 0;
@@ -339,7 +329,8 @@ struct Part {
         HIPCHK(hipGraphDestroy(g));
         return ex;
     }
-    void ensure_lists() {
+    // Rebuilds the lists of the replicas marked pending (host-forced).
+    void rebuild_pending() {
         bool any = false;
         for (char c : pending_rb) any |= c != 0;
         if (!any) return;
@@ -349,7 +340,109 @@ struct Part {
         k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, cur, 0);
         rebuild_seq(D.pos[cur], cur);
         std::fill(pending_rb.begin(), pending_rb.end(), 0);
+    }
+    void ensure_lists() {
+        rebuild_pending();
+        settle();
+    }
+    // After the stream has drained: widen the rows for replicas halted by
+    // a row overflow alone (grow_rows), then report any error left.
+    void settle() {
+        while (grow_rows()) {
+        }
         check_err();
+    }
+    // ---------------------------------------------------- row growth ---
+    // Replicas halted by E_NL_CAP alone (a rebuild found a row longer than
+    // the row width; the replica's state is that at the start of the next
+    // step): every row is widened in place of the old ones (the other
+    // replicas' rows keep their contents), those replicas are rebuilt from
+    // the positions they halted at -- the lists an engine with the wider
+    // rows built at that step -- and each then steps alone, the others
+    // held, until it has taken as many steps as they have.  False when
+    // there is nothing to grow or the rows would exceed MAXSORT.
+    bool grow_rows() {
+        HIPCHK(hipStreamSynchronize(stream));
+        auto e = get(D.err, nrep);
+        std::vector<int> rec;
+        for (int k = 0; k < nrep; ++k)
+            if (e[k] == E_NL_CAP) rec.push_back(k);
+        if (rec.empty()) return false;
+        auto need = get(D.need_w, nrep);
+        int want = 0;
+        for (int k : rec) want = std::max(want, need[k]);
+        if (want > MAXSORT) return false;
+        widen_rows(std::min(MAXSORT, round_up(std::max(want + want / 4, nl_w + WAVE), WAVE)));
+        put(D.need_w, std::vector<int>(nrep, 0));
+        auto rb = get(D.n_rebuild, nrep);
+        for (int k : rec) {
+            clear_err(k);
+            pending_rb[k] = 1;
+            rb[k] -= 1;  // the rebuild that overflowed; the one below counts
+        }
+        put(D.n_rebuild, rb);
+        rebuild_pending();
+        for (int k : rec) catch_up(k);
+        return true;
+    }
+    void widen_rows(int W2) {
+        const int nsl = nslot / WAVE;
+        if ((long long)nslot * W2 >= (1LL << 31)) throw Fail{XMD_ERR_CAPACITY, "rows too wide for int32 offsets"};
+        int *nj = alloc<int>((size_t)nslot * W2);
+        uint8_t *nf = alloc<uint8_t>((size_t)nslot * W2);
+        HIPCHK(hipMemcpy2DAsync(nj, (size_t)WAVE * W2 * sizeof(int), D.nl_j, (size_t)WAVE * nl_w * sizeof(int),
+                                (size_t)WAVE * nl_w * sizeof(int), nsl, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpy2DAsync(nf, (size_t)WAVE * W2, D.nl_flag, (size_t)WAVE * nl_w, (size_t)WAVE * nl_w, nsl,
+                                hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        release(D.nl_j);
+        release(D.nl_flag);
+        D.nl_j = nj;
+        D.nl_flag = nf;
+        D.nl_w = nl_w = W2;
+        std::vector<int> ss(nsl);
+        for (int sl = 0; sl < nsl; ++sl) ss[sl] = sl * WAVE * W2;
+        put(D.slice_start, ss);
+        nl_total = (long long)nslot * W2;
+        drop_graphs();  // they hold the old Dev
+    }
+    // Steps replica k alone until it has taken as many steps as the engine
+    // (every step of this part not spent catching a replica up); the
+    // others are held meanwhile.
+    void catch_up(int k) {
+        auto done = get(D.n_done, nrep);
+        auto e = get(D.err, nrep);
+        long long lag = (steps - catchup_steps) - done[k];
+        if (lag <= 0 || e[k]) return;
+        std::vector<int> h = e;
+        for (int j = 0; j < nrep; ++j)
+            if (j != k && !e[j]) h[j] = E_HOLD;
+        put(D.err, h);
+        catchup_steps += lag;
+        while (lag > 0) lag -= launch_next(lag);
+        drain();
+        auto after = get(D.err, nrep);
+        for (int j = 0; j < nrep; ++j)
+            if (h[j] == E_HOLD) after[j] = 0;
+        put(D.err, after);
+    }
+    void release(void *p) {
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) allocs.erase(it);
+        HIPCHK(hipFree(p));
+    }
+    void drop_graphs() {
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto &g : g_chunk)
+            if (g) {
+                HIPCHK(hipGraphExecDestroy(g));
+                g = nullptr;
+            }
+        for (auto &g : g_one)
+            if (g) {
+                HIPCHK(hipGraphExecDestroy(g));
+                g = nullptr;
+            }
     }
     // Error words: one per replica.  A kernel that finds an error sets its
     // replica's word; every kernel then leaves that replica as it is
@@ -402,6 +495,13 @@ struct Part {
         const int zero = 0;
         HIPCHK(hipMemcpy(D.err + rep, &zero, sizeof(int), hipMemcpyHostToDevice));
         reported[rep] = 0;
+    }
+    // A state set from the host: the replica is running again, level with
+    // the engine's step count.
+    void reset_replica(int rep) {
+        clear_err(rep);
+        const long long done = steps - catchup_steps;
+        HIPCHK(hipMemcpy(D.n_done + rep, &done, sizeof(long long), hipMemcpyHostToDevice));
     }
     // Waits for the oldest queued graph launch when MAX_INFLIGHT are queued.
     void throttle() {
@@ -864,6 +964,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.any_rb = E.alloc<int>(2);
     D.err = E.alloc<int>(E.nrep);
     E.reported.assign(E.nrep, 0);
+    D.n_done = E.alloc<long long>(E.nrep);
+    D.need_w = E.alloc<int>(E.nrep);
     D.n_rebuild = E.alloc<long long>(E.nrep);
     D.cell_off = E.upload(cell_off);
     D.cell_dim = E.upload(cell_dim);
@@ -992,7 +1094,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     // (chains compact after the start), at least 256, at most the LDS sort
     // buffer; a longer row later is a capacity error
     if (maxlen > MAXSORT) throw Fail{XMD_ERR_CAPACITY, "a neighbour row is longer than the largest supported row"};
-    E.nl_w = std::min(MAXSORT, std::max(256, round_up(3 * maxlen + 64, WAVE)));
+    if (P.row_width > 0) E.nl_w = std::min(MAXSORT, std::max(P.row_width, std::max(maxlen, 1)));
+    else E.nl_w = std::min(MAXSORT, std::max(256, round_up(3 * maxlen + 64, WAVE)));
     D.nl_w = E.nl_w;
     {
         std::vector<int> ss(nsl);
@@ -1113,7 +1216,7 @@ static xmd_status part_set_state(Part *e, int32_t rep, const double *y, uint64_t
             }
         HIPCHK(hipMemcpy(e->D.rng[e->cur] + rep, &rng, sizeof(uint64_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->D.tt[e->cur] + rep, &t, sizeof(double), hipMemcpyHostToDevice));
-        e->clear_err(rep);
+        e->reset_replica(rep);
         e->pending_rb[rep] = 1;
     });
 }
@@ -1151,7 +1254,7 @@ static xmd_status part_set_positions(Part *e, int32_t rep, const double *r) {
         int nb = e->rep_n[rep], s0 = e->rep_slot_off[rep];
         HIPCHK(hipStreamSynchronize(e->stream));
         put_positions(e, s0, nb, r);
-        e->clear_err(rep);
+        e->reset_replica(rep);
         e->pending_rb[rep] = 1;
     });
 }
@@ -1404,7 +1507,7 @@ static xmd_status part_set_qa_state(Part *e, int32_t rep, const xmd_qa_contact *
             HIPCHK(hipMemcpy(e->D.qslot + (size_t)q * n + s0, qslot.data() + (size_t)q * nb, nb * sizeof(int),
                              hipMemcpyHostToDevice));
         if (sync) HIPCHK(hipMemcpy(e->D.sync + s0, sync, 4 * (size_t)nb, hipMemcpyHostToDevice));
-        e->clear_err(rep);
+        e->reset_replica(rep);
         e->pending_rb[rep] = 1;
     });
 }
@@ -1436,7 +1539,7 @@ static xmd_status part_get_counters(Part *e, int64_t *out) {
             auto v = e->get(e->D.ct_vac + e->ct_base_h[k], cnt[k]);
             for (auto x : v) qa += !x;
         }
-        out[0] = e->steps;
+        out[0] = e->steps - e->catchup_steps;
         out[1] = r;
         out[2] = nl;
         out[3] = e->nl_total;
@@ -1652,6 +1755,7 @@ void validate(const xmd_params *p, const xmd_topology *t) {
     if (p->force_lanes != 0 && p->force_lanes != 4 && p->force_lanes != 8 && p->force_lanes != 16)
         bad("force_lanes must be 0, 4, 8 or 16");
     if (p->stream_groups < 0) bad("stream_groups must be >= 0");
+    if (p->row_width < 0) bad("row_width must be >= 0");
     if (t->n_replicas <= 0 || t->n_beads <= 0 || !t->replica_offset || !t->chain_offset || !t->r0)
         bad("empty topology");
     if (!t->atype || !t->box || !t->temperature || !t->rng_state) bad("null topology array");
@@ -1880,7 +1984,7 @@ xmd_status xmd_step(xmd_engine *e, int64_t nsteps) {
     for (Part *q : e->parts) {
         xmd_status st = run_guarded(q->err, [&]() {
             q->drain();
-            q->check_err();
+            q->settle();
         });
         if (st != XMD_OK) {
             if (first == XMD_OK) first = st;

@@ -528,6 +528,7 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
         if (l == 0) {
             D.rng[1 - p][rep] = Sn;
             D.tt[1 - p][rep] = D.tt[p][rep] + G.dt;
+            D.n_done[rep] += 1;
         }
     }
     if (l == 0) {
@@ -785,7 +786,10 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
             cnt += __popcll(m);
         });
         if (cnt > W) {
-            if (l == 0) atomicOr(D.err + rep, E_NL_CAP);
+            if (l == 0) {
+                atomicOr(D.err + rep, E_NL_CAP);
+                atomicMax(D.need_w + rep, cnt);
+            }
             cnt = W;
         }
     }
