@@ -202,6 +202,13 @@ typedef struct xmd_params {
        the wider rows).  No counterpart in freeform_params, whose lists are
        vectors that grow (nl/legacy_update.cpp:28-59). */
     int32_t row_width;
+    /* initial capacity of each replica's QA free-pair set: 0 = automatic
+       (the first list's pairs, doubled, plus the contact capacity), n > 0 =
+       n rounded up to 256.  A rebuild that finds more free pairs moves
+       every replica's set into larger regions and steps the replica that
+       needed it alone back to the others, as for the rows.  No counterpart
+       in freeform_params (qa.free_pairs is a vector, qa/free_pair.cpp). */
+    int32_t fp_capacity;
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and

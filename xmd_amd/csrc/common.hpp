@@ -61,6 +61,7 @@ struct Dev {
     long long *n_rebuild;
     long long *n_done;  // steps each replica has taken (halted ones stop counting)
     int *need_w;        // longest row a rebuild of the replica wanted beyond the row width
+    int *need_fp;       // QA free pairs a rebuild of the replica found beyond its capacity
     const int *cell_off, *cell_dim;  // [nrep+1], [3*nrep]
     int *cell_cnt, *cell_start, *cell_end, *cell_cursor, *cell_beads, *bead_cell;
     // slots

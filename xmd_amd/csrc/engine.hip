@@ -345,35 +345,52 @@ struct Part {
         rebuild_pending();
         settle();
     }
-    // After the stream has drained: widen the rows for replicas halted by
-    // a row overflow alone (grow_rows), then report any error left.
+    // After the stream has drained: grow the lists of replicas halted by a
+    // rebuild that outgrew them (grow_lists), then report any error left.
     void settle() {
-        while (grow_rows()) {
+        while (grow_lists()) {
         }
         check_err();
     }
-    // ---------------------------------------------------- row growth ---
-    // Replicas halted by E_NL_CAP alone (a rebuild found a row longer than
-    // the row width; the replica's state is that at the start of the next
-    // step): every row is widened in place of the old ones (the other
-    // replicas' rows keep their contents), those replicas are rebuilt from
-    // the positions they halted at -- the lists an engine with the wider
-    // rows built at that step -- and each then steps alone, the others
-    // held, until it has taken as many steps as they have.  False when
-    // there is nothing to grow or the rows would exceed MAXSORT.
-    bool grow_rows() {
+    // --------------------------------------------------- list growth ---
+    // Replicas halted by a list rebuild that outgrew its capacity and by
+    // nothing else -- a row longer than the row width (E_NL_CAP, with the
+    // width it wanted) or more QA free pairs than the replica's region
+    // holds (E_FP_CAP, with the count it found) -- have the state at the
+    // start of the next step, with incomplete lists.  The rows are widened
+    // in place of the old ones and the free-pair regions moved into larger
+    // ones (the other replicas' lists keep their contents); those replicas
+    // are rebuilt from the positions they halted at -- the lists an engine
+    // created with the larger capacities built at that step -- and each
+    // then steps alone, the others held, until it has taken as many steps
+    // as they have.  False when there is nothing to grow (a row beyond
+    // MAXSORT, or a free-pair overflow of the QA pass, stays an error).
+    bool grow_lists() {
         HIPCHK(hipStreamSynchronize(stream));
         auto e = get(D.err, nrep);
-        std::vector<int> rec;
-        for (int k = 0; k < nrep; ++k)
-            if (e[k] == E_NL_CAP) rec.push_back(k);
-        if (rec.empty()) return false;
-        auto need = get(D.need_w, nrep);
+        auto nw = get(D.need_w, nrep);
+        auto nf = get(D.need_fp, nrep);
+        std::vector<int> rec, cap2(fp_cap_h.begin(), fp_cap_h.begin() + nrep);
         int want = 0;
-        for (int k : rec) want = std::max(want, need[k]);
-        if (want > MAXSORT) return false;
-        widen_rows(std::min(MAXSORT, round_up(std::max(want + want / 4, nl_w + WAVE), WAVE)));
+        bool fp_grow = false;
+        for (int k = 0; k < nrep; ++k) {
+            if (!e[k] || (e[k] & ~(E_NL_CAP | E_FP_CAP))) continue;
+            if ((e[k] & E_NL_CAP) && nw[k] > MAXSORT) continue;
+            if ((e[k] & E_FP_CAP) && nf[k] <= 0) continue;
+            rec.push_back(k);
+            if (e[k] & E_NL_CAP) want = std::max(want, nw[k]);
+            if (e[k] & E_FP_CAP) {
+                const long long c = std::max<long long>(nf[k] + nf[k] / 4LL, 2LL * fp_cap_h[k]);
+                if (c >= (1LL << 30)) throw Fail{XMD_ERR_CAPACITY, "too many QA free pairs in one replica"};
+                cap2[k] = round_up((int)c, BLK);
+                fp_grow = true;
+            }
+        }
+        if (rec.empty()) return false;
+        if (want > 0) widen_rows(std::min(MAXSORT, round_up(std::max(want + want / 4, nl_w + WAVE), WAVE)));
+        if (fp_grow) grow_free_pairs(cap2);
         put(D.need_w, std::vector<int>(nrep, 0));
+        put(D.need_fp, std::vector<int>(nrep, 0));
         auto rb = get(D.n_rebuild, nrep);
         for (int k : rec) {
             clear_err(k);
@@ -384,6 +401,52 @@ struct Part {
         rebuild_pending();
         for (int k : rec) catch_up(k);
         return true;
+    }
+    // Moves every replica's QA free pairs (pairs and vacancy flags) into
+    // regions of `cap` entries each (multiples of BLK, none smaller than
+    // before); the per-step candidate arrays start empty.
+    void grow_free_pairs(const std::vector<int> &cap) {
+        std::vector<int> base(nrep + 1), capv(nrep + 1);
+        long long fp = 0;
+        for (int k = 0; k < nrep; ++k) {
+            base[k] = (int)fp;
+            capv[k] = cap[k];
+            fp += cap[k];
+        }
+        if (fp >= (1LL << 31)) throw Fail{XMD_ERR_CAPACITY, "too many QA free pairs for int32 offsets"};
+        base[nrep] = capv[nrep] = (int)fp;
+        int *i1 = alloc<int>(fp), *i2 = alloc<int>(fp);
+        uint8_t *vac = alloc<uint8_t>(fp);
+        for (int k = 0; k < nrep; ++k) {
+            const size_t o = fp_base_h[k], n = fp_cap_h[k], b = base[k];
+            HIPCHK(hipMemcpyAsync(i1 + b, D.fp_i1 + o, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(i2 + b, D.fp_i2 + o, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(vac + b, D.fp_vac + o, n, hipMemcpyDeviceToDevice, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        for (void *q : {(void *)D.fp_i1, (void *)D.fp_i2, (void *)D.fp_vac, (void *)D.cand_cnt, (void *)D.cand_fp,
+                        (void *)D.cand_type, (void *)D.cand_d1, (void *)D.cand_d2, (void *)D.fp_chunk_rep})
+            release(q);
+        D.fp_i1 = i1;
+        D.fp_i2 = i2;
+        D.fp_vac = vac;
+        D.cand_cnt = alloc<int>(fp / BLK);
+        D.cand_fp = alloc<int>(fp);
+        D.cand_type = alloc<int>(fp);
+        D.cand_d1 = alloc<uint32_t>(fp);
+        D.cand_d2 = alloc<uint32_t>(fp);
+        std::vector<int> fprep(fp / BLK, -1);
+        for (int k = 0; k < nrep; ++k)
+            for (int c = base[k] / BLK; c < (base[k] + capv[k]) / BLK; ++c) fprep[c] = k;
+        D.fp_chunk_rep = upload(fprep);
+        fp_base_h = base;
+        fp_cap_h = capv;
+        put(const_cast<int *>(D.fp_base), base);
+        put(const_cast<int *>(D.fp_cap), capv);
+        fp_total = (int)fp;
+        nfp_chunks = (int)(fp / BLK);
+        qa_grid = std::max(1, std::min(nfp_chunks + nct_chunks, 8192));
+        drop_graphs();  // they hold the old Dev
     }
     void widen_rows(int W2) {
         const int nsl = nslot / WAVE;
@@ -966,6 +1029,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     E.reported.assign(E.nrep, 0);
     D.n_done = E.alloc<long long>(E.nrep);
     D.need_w = E.alloc<int>(E.nrep);
+    D.need_fp = E.alloc<int>(E.nrep);
     D.n_rebuild = E.alloc<long long>(E.nrep);
     D.cell_off = E.upload(cell_off);
     D.cell_dim = E.upload(cell_dim);
@@ -1110,7 +1174,9 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
         for (int sl = E.rep_slot_off[k] / WAVE; sl < (E.rep_slot_off[k] + E.rep_n[k] + WAVE - 1) / WAVE; ++sl)
             need += 64LL * w[sl];
         long long cap = round_up((int)std::min<long long>(2 * need + 64LL * 64, 1LL << 30), BLK);
-        long long fcap = P.qa_enabled ? round_up((int)(cap / 2 + E.ct_cap_h[k]), BLK) : BLK;
+        long long fcap = !P.qa_enabled ? BLK
+                         : P.fp_capacity > 0 ? round_up(std::min(P.fp_capacity, 1 << 29), BLK)
+                                             : round_up((int)(cap / 2 + E.ct_cap_h[k]), BLK);
         E.fp_base_h[k] = (int)fp;
         E.fp_cap_h[k] = (int)fcap;
         fp += fcap;
@@ -1147,7 +1213,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     k_fp_write<<<E.nl_grid(), BLK, 0, E.stream>>>(D, 0);
     HIPCHK(hipGetLastError());
     std::fill(E.pending_rb.begin(), E.pending_rb.end(), 0);
-    E.check_err();
+    E.settle();  // a first list beyond a given fp_capacity grows it
 }
 
 xmd_status run_guarded(std::string &err, const std::function<void()> &fn) {
@@ -1756,6 +1822,7 @@ void validate(const xmd_params *p, const xmd_topology *t) {
         bad("force_lanes must be 0, 4, 8 or 16");
     if (p->stream_groups < 0) bad("stream_groups must be >= 0");
     if (p->row_width < 0) bad("row_width must be >= 0");
+    if (p->fp_capacity < 0) bad("fp_capacity must be >= 0");
     if (t->n_replicas <= 0 || t->n_beads <= 0 || !t->replica_offset || !t->chain_offset || !t->r0)
         bad("empty topology");
     if (!t->atype || !t->box || !t->temperature || !t->rng_state) bad("null topology array");

@@ -624,7 +624,12 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
     if (tid == 0) {
         int used = s_carry - base;
         if (mode == 2) {
-            if (used > D.fp_cap[rep]) atomicOr(D.err + rep, E_FP_CAP);
+            if (used > D.fp_cap[rep]) {
+                // the count the host grows the replica's region to (the
+                // overflow of the QA pass leaves it 0: that one halts)
+                atomicOr(D.err + rep, E_FP_CAP);
+                D.need_fp[rep] = used;
+            }
             D.fp_count[rep] = used > D.fp_cap[rep] ? D.fp_cap[rep] : used;
         }
     }
