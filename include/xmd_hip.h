@@ -393,10 +393,12 @@ xmd_status xmd_observables(xmd_engine *eng, double active_threshold, double *out
 xmd_status xmd_observables_noalloc(xmd_engine *eng, double active_threshold, double *out);
 
 /* Autotuner (no counterpart in the reference): times `reps` launches of the
- * force kernel with 4, 8 and 16 lanes per bead on the current state, keeps
- * the fastest for every later step and returns it in *lpb_out and the three
- * times (ms per step, summed over stream groups) in ms_out[3] (either may be
- * NULL).  The simulation state is not changed. */
+ * force kernel with 4, 8 and 16 lanes per bead on the current state, every
+ * stream group's launches side by side as in a step, in three interleaved
+ * rounds; keeps the candidate with the lowest median for every later step
+ * and returns it in *lpb_out and the three medians (ms per step: the span of
+ * one launch of every group) in ms_out[3] (either may be NULL).  The
+ * simulation state is not changed. */
 xmd_status xmd_autotune(xmd_engine *eng, int32_t reps, int32_t *lpb_out, double *ms_out);
 #ifdef __cplusplus
 }

@@ -111,7 +111,8 @@ struct Dev {
     const double *ss_depth, *ss_rmin, *ss_rmax, *req_min_dist, *charge;
     const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
     const int8_t *ptype;
-    const uint8_t *bm_fix;
+    const uint8_t *bm_fix;  // glibc correction table and its thresholds (bead_noise)
+    const float *bm_thr;
     // heuristic bonded tables by pair type 3 c1 + c2 (c: 0 GLY, 1 PRO, 2
     // other): 7 angle polynomial coefficients, then 6 dihedral coefficients
     const double *heur_ang_c, *heur_dih_c;

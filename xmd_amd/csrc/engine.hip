@@ -84,9 +84,15 @@ void rows_of(const Mat &M, uint64_t *rows) {
 
 // --------------------------------------------------- glibc fix table ---
 // One byte per reachable draw value: corrections of logf / sinf / cosf
-// (bm_math.hpp).  Built once per process with the host's glibc.
+// (bm_math.hpp), and per bucket of 2^FIX_BUCKET_SHIFT draw values the
+// margins below which a draw may need one (bead_noise).  Built once per
+// process with the host's glibc.
+struct FixTable {
+    std::vector<uint8_t> tab;
+    std::vector<float> thr;  // [bucket][log, sin, cos]
+};
 std::mutex g_fix_mu;
-std::vector<uint8_t> g_fix;
+FixTable g_fix;
 
 uint8_t fix_code(float mine, float ref) {
     if (mine == ref || (mine != mine && ref != ref)) return 0;
@@ -95,34 +101,46 @@ uint8_t fix_code(float mine, float ref) {
     throw Fail{XMD_ERR_STATE, "glibc libm differs from the engine's by more than one ulp"};
 }
 
-const std::vector<uint8_t> &fix_table() {
+// a / spacing of boundary_margin (exact: the spacing is a power of two)
+double margin_ratio(double y, float f) {
+    double a, sp;
+    boundary_margin(y, f, a, sp);
+    return a / sp;
+}
+
+const FixTable &fix_table() {
     std::lock_guard<std::mutex> lk(g_fix_mu);
-    if (!g_fix.empty()) return g_fix;
-    const uint32_t N = xmd_bm::NUM_DRAW_VALUES;
+    if (!g_fix.tab.empty()) return g_fix;
+    const uint32_t N = xmd_bm::NUM_DRAW_VALUES, NB = (N >> FIX_BUCKET_SHIFT) + 1;
     std::vector<uint8_t> tab(N);
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;
     std::vector<int> bad(nt, 0);
-    const float two_pi = (float)(2.0 * M_PI);
+    std::vector<std::vector<double>> mx(nt, std::vector<double>(3 * (size_t)NB, -1.0));
     for (unsigned t = 0; t < nt; ++t) {
         th.emplace_back([&, t]() {
             try {
                 for (uint32_t k = t; k < N; k += nt) {
-                    // value with draw_index(v) == k
+                    // the draw's float value v with draw_index(v) == k
                     float v;
                     if (k < 16777216u) v = (float)k;
                     else {
                         uint32_t q = k - 16777216u, e = q >> 23, m = q & 0x7fffffu;
                         v = xmd_bm::bitsf(((151u + e) << 23) | m);
                     }
-                    float x = v * (1.0f / 4294967296.0f);
-                    uint8_t c = fix_code(xmd_bm::logf_cr(x), logf(x));
-                    float tt = two_pi * x;
+                    const float x = v * (1.0f / 4294967296.0f);
+                    const float two_pi_x = (float)(2.0 * M_PI) * x;
+                    const double yl = bm_log_d(v);
                     double sd, cd;
-                    xmd_bm::sincos_red((double)tt, sd, cd);
-                    c |= (uint8_t)(fix_code((float)sd, sinf(tt)) << 2);
-                    c |= (uint8_t)(fix_code((float)cd, cosf(tt)) << 4);
-                    tab[k] = c;
+                    bm_sincos_d(v, sd, cd);
+                    const float fl = (float)yl, fs = (float)sd, fc = (float)cd;
+                    const uint8_t cl = fix_code(fl, logf(x)), cs = fix_code(fs, sinf(two_pi_x)),
+                                  cc = fix_code(fc, cosf(two_pi_x));
+                    tab[k] = (uint8_t)(cl | (cs << 2) | (cc << 4));
+                    double *m = mx[t].data() + 3 * (size_t)(k >> FIX_BUCKET_SHIFT);
+                    if (cl) m[0] = std::max(m[0], margin_ratio(yl, fl));
+                    if (cs) m[1] = std::max(m[1], margin_ratio(sd, fs));
+                    if (cc) m[2] = std::max(m[2], margin_ratio(cd, fc));
                 }
             } catch (...) {
                 bad[t] = 1;
@@ -132,7 +150,16 @@ const std::vector<uint8_t> &fix_table() {
     for (auto &x : th) x.join();
     for (int b : bad)
         if (b) throw Fail{XMD_ERR_STATE, "glibc libm differs from the engine's by more than one ulp"};
-    g_fix.swap(tab);
+    std::vector<float> thr(3 * (size_t)NB);
+    for (size_t j = 0; j < thr.size(); ++j) {
+        double m = -1.0;
+        for (unsigned t = 0; t < nt; ++t) m = std::max(m, mx[t][j]);
+        float f = (float)m;
+        if ((double)f < m) f = xmd_bm::next_up(f);  // rounded up: the test stays conservative
+        thr[j] = f;
+    }
+    g_fix.tab.swap(tab);
+    g_fix.thr.swap(thr);
     return g_fix;
 }
 
@@ -169,26 +196,16 @@ struct Part {
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr}, g_one[2] = {nullptr, nullptr};
     int chunk_steps = 16;
     int lpb = FORCE_LPB;  // lanes per bead of the force kernel
-    // Mean duration (ms) of `reps` force-kernel launches with `l` lanes per
-    // bead on the current state.  The kernel only writes the force buffers,
-    // which every step recomputes, so the simulation state is untouched.
-    double time_force(int l, int reps) {
-        ensure_lists();
-        hipEvent_t a, b;
-        HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
+    // Queues `reps` force-kernel launches with `l` lanes per bead on the
+    // current state between events a and b (recorded on the part's stream).
+    // The kernel only writes the force buffers, which every step
+    // recomputes, so the simulation state is untouched.
+    void queue_force(int l, int reps, hipEvent_t a, hipEvent_t b) {
         const bool mixed = P.precision == XMD_PREC_MIXED;
-        launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur, 0);  // warm-up
         HIPCHK(hipEventRecord(a, stream));
         for (int k = 0; k < reps; ++k) launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur, 0);
         HIPCHK(hipEventRecord(b, stream));
-        HIPCHK(hipEventSynchronize(b));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, a, b));
-        HIPCHK(hipEventDestroy(a));
-        HIPCHK(hipEventDestroy(b));
         HIPCHK(hipGetLastError());
-        return ms / reps;
     }
     // Use `l` lanes per bead from now on (graphs are captured again).
     void set_lpb(int l) {
@@ -1092,7 +1109,11 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.jwave_n = jwave_n;
     D.jnib = E.upload(jnib);
     D.jadv = E.upload(jadv);
-    if (P.rng_exact) D.bm_fix = E.upload(fix_table());
+    if (P.rng_exact) {
+        const FixTable &F = fix_table();
+        D.bm_fix = E.upload(F.tab);
+        D.bm_thr = E.upload(F.thr);
+    }
     int nsl = n / WAVE;
     D.slice_start = E.alloc<int>(nsl);
     D.nl_len = E.alloc<int>(n);
@@ -2200,22 +2221,54 @@ xmd_status xmd_observables_noalloc(xmd_engine *e, double active_threshold, doubl
 xmd_status xmd_autotune(xmd_engine *e, int32_t reps, int32_t *lpb_out, double *ms_out) {
     if (!e || reps <= 0) return XMD_ERR_ARG;
     static const int cand[3] = {4, 8, 16};
+    constexpr int ROUNDS = 3;
+    const size_t np = e->parts.size();
+    // every stream group's launches run side by side, as in a step; the
+    // candidates are timed in interleaved rounds and each keeps its median
+    // (the span from the first group's start to the last group's end)
+    std::vector<double> t[3];
+    std::vector<hipEvent_t> ev(2 * np, nullptr);
+    xmd_status st = XMD_OK;
+    for (size_t g = 0; g < np && st == XMD_OK; ++g)
+        st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
+                         e->parts[g]->ensure_lists();
+                         HIPCHK(hipEventCreate(&ev[2 * g]));
+                         HIPCHK(hipEventCreate(&ev[2 * g + 1]));
+                     }));
+    for (int r = -1; r < ROUNDS && st == XMD_OK; ++r)  // round -1 warms up
+        for (int c = 0; c < 3 && st == XMD_OK; ++c) {
+            for (size_t g = 0; g < np && st == XMD_OK; ++g)
+                st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
+                                 e->parts[g]->queue_force(cand[c], r < 0 ? 1 : reps, ev[2 * g], ev[2 * g + 1]);
+                             }));
+            double lo = 0.0, hi = 0.0;
+            for (size_t g = 0; g < np && st == XMD_OK; ++g)
+                st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
+                                 HIPCHK(hipEventSynchronize(ev[2 * g + 1]));
+                                 float ms0 = 0.0f, ms1 = 0.0f;
+                                 HIPCHK(hipEventElapsedTime(&ms0, ev[0], ev[2 * g]));
+                                 HIPCHK(hipEventElapsedTime(&ms1, ev[0], ev[2 * g + 1]));
+                                 lo = g == 0 ? ms0 : std::min(lo, (double)ms0);
+                                 hi = g == 0 ? ms1 : std::max(hi, (double)ms1);
+                             }));
+            if (r >= 0) t[c].push_back((hi - lo) / reps);
+        }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (st != XMD_OK) return st;
     double best = 0.0;
     int pick = FORCE_LPB;
     for (int c = 0; c < 3; ++c) {
-        double ms = 0.0;
-        for (Part *q : e->parts) {
-            xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { ms += q->time_force(cand[c], reps); }));
-            if (st != XMD_OK) return st;
-        }
-        if (ms_out) ms_out[c] = ms;
-        if (c == 0 || ms < best) {
-            best = ms;
+        std::sort(t[c].begin(), t[c].end());
+        const double med = t[c][ROUNDS / 2];
+        if (ms_out) ms_out[c] = med;
+        if (c == 0 || med < best) {
+            best = med;
             pick = cand[c];
         }
     }
     for (Part *q : e->parts) {
-        xmd_status st = on_part(e, q, run_guarded(q->err, [&]() { q->set_lpb(pick); }));
+        st = on_part(e, q, run_guarded(q->err, [&]() { q->set_lpb(pick); }));
         if (st != XMD_OK) return st;
     }
     if (lpb_out) *lpb_out = pick;

@@ -394,7 +394,8 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
     const int n = D.nslot;
     double *y = D.y;
     // the bead's state first: these loads overlap the noise's dependent
-    // chain (generator state -> jump tables -> correction table)
+    // chain (generator state -> jump tables -> log, sin, cos -> the
+    // correction table for the few draws near a rounding boundary)
     v3<double> y0{0, 0, 0}, y1{0, 0, 0}, y2{0, 0, 0}, y3{0, 0, 0}, y4{0, 0, 0}, y5{0, 0, 0}, r_orig{0, 0, 0};
     if (active) {
         y0 = ldp(D.pos[p], s);
@@ -407,7 +408,7 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
     int local = s - D.rep_slot_off[rep];
     uint64_t S = wave_lane_state(D, D.rng[p][rep], local >> 6);
     float nz[3];
-    bead_noise(S, D.bm_fix, nz);
+    bead_noise(S, NoiseFix{D.bm_fix, D.bm_thr}, nz);
     // a zero uniform in a log() slot gives infinite noise (rand_gen.h:45-46)
     if (active && (!((rng_out(S) >> 32)) || !((rng_out(rng_step(rng_step(S))) >> 32)))) atomicOr(D.err + rep, E_ZERO_DRAW);
     double disp = 0.0;
@@ -1025,7 +1026,7 @@ __global__ __launch_bounds__(BLK) void k_rand_normals(Dev D, uint64_t S, long lo
     int w = (int)(k >> 6);
     uint64_t st = wave_lane_state(D, S, w);
     float nz[3];
-    bead_noise(st, D.bm_fix, nz);
+    bead_noise(st, NoiseFix{D.bm_fix, D.bm_thr}, nz);
     if (k < nb) {
         out[3 * k] = nz[0];
         out[3 * k + 1] = nz[1];

@@ -700,44 +700,95 @@ XMD_HD inline uint64_t rng_step(uint64_t s) {
     return s;
 }
 // normal_x2 / normal of rand_gen (rand_gen.h:42-58) with the corrected
-// float libm (bm_math.hpp); fix_table[k] holds the glibc corrections of
-// draw value k (bits 0-1 log, 2-3 sin, 4-5 cos), or nullptr to skip them.
-// `code` is the draw's byte of the fix table (0 = no correction).
-XMD_HD inline float bm_radius(uint32_t hi, uint32_t code) {
-    float v = (float)hi;
-    float r1 = v * (1.0f / 4294967296.0f);
-    float lg = xmd_bm::apply_fix(xmd_bm::logf_cr(r1), code & 3u);
+// float libm (bm_math.hpp).  The correction table holds, for each draw value
+// k, glibc's corrections of log, sin and cos of that draw (bits 0-1 log,
+// 2-3 sin, 4-5 cos; 0 keep, 1 one ulp up, 2 one ulp down).
+//
+// The float f = (float)y of the engine's double evaluation y can differ from
+// glibc's only where y lies near the rounding boundary between f and its
+// neighbour on y's side.  a = (half their spacing) - |y - f| measures that
+// distance; thr[3 b + j] (j = 0 log, 1 sin, 2 cos) is the largest a /
+// spacing among the draw values of bucket b = k >> FIX_BUCKET_SHIFT where
+// glibc's function j differs from f, rounded up to float, found on the host
+// over every draw value with this same function.  So a draw whose margins
+// are all above its bucket's thresholds needs no correction, and only the
+// others read the table (about 2 % of log draws and 6 % of angle draws).
+constexpr int FIX_BUCKET_SHIFT = 16;
+struct NoiseFix {
+    const uint8_t *tab;  // nullptr: no correction (correctly rounded noise)
+    const float *thr;
+};
+// a and the spacing (a power of two, so a <= T * spacing is exact); a
+// non-finite f gives a = -1, spacing 1 (always below a threshold >= -1).
+XMD_HD inline void boundary_margin(double y, float f, double &a, double &sp) {
+    if (!(f - f == 0.0f)) {
+        a = -1.0;
+        sp = 1.0;
+        return;
+    }
+    const double df = (double)f;
+    const float g = y > df ? xmd_bm::next_up(f) : xmd_bm::next_down(f);
+    sp = fabs((double)g - df);
+    a = 0.5 * sp - fabs(y - df);
+}
+XMD_HD inline bool near_boundary(double y, float f, float T) {
+    double a, sp;
+    boundary_margin(y, f, a, sp);
+    return a <= (double)T * sp;
+}
+// log of a radius draw (rand_gen.h:47) and sin, cos of an angle draw
+// (rand_gen.h:48-49) in double, before rounding to float; v = (float) of the
+// draw's upper 32 bits (up to 2^32, which the rounding reaches)
+XMD_HD inline double bm_log_d(float v) {
+    const float r1 = v * (1.0f / 4294967296.0f);
+    return r1 == 0.0f ? -(double)__builtin_inff() : xmd_bm::log_d((double)r1);
+}
+XMD_HD inline void bm_sincos_d(float v, double &sd, double &cd) {
+    const float r2 = v * (1.0f / 4294967296.0f);
+    const float t = (float)(2.0 * M_PI) * r2;
+    xmd_bm::sincos_red((double)t, sd, cd);
+}
+XMD_HD inline float bm_radius(float lg) {
     float arg = -2.0f * lg;
     float r = (float)sqrt((double)arg);
     if (r != r) r = 0.0f;
     return r;
 }
-XMD_HD inline void bm_angle(uint32_t hi, uint32_t code, float &s, float &c) {
-    float v = (float)hi;
-    float r2 = v * (1.0f / 4294967296.0f);
-    float t = (float)(2.0 * M_PI) * r2;
-    double sd, cd;
-    xmd_bm::sincos_red((double)t, sd, cd);
-    s = xmd_bm::apply_fix((float)sd, (code >> 2) & 3u);
-    c = xmd_bm::apply_fix((float)cd, (code >> 4) & 3u);
-}
-// The three noise components of one bead from its first state.  The four
-// fix-table bytes are fetched before any arithmetic so their loads overlap.
-XMD_HD inline void bead_noise(uint64_t s, const uint8_t *fix, float out[3]) {
-    uint32_t u[4], code[4] = {0u, 0u, 0u, 0u};
+// The three noise components of one bead from its first state: normal_x2
+// from draws 0 (radius) and 1 (angle), normal from draws 2 and 3 (its cosine
+// only).
+XMD_HD inline void bead_noise(uint64_t s, NoiseFix fx, float out[3]) {
+    float u[4];
     for (int k = 0; k < 4; ++k) {
-        u[k] = (uint32_t)(rng_out(s) >> 32);
+        u[k] = (float)(uint32_t)(rng_out(s) >> 32);
         s = rng_step(s);
     }
-    if (fix)
-        for (int k = 0; k < 4; ++k) code[k] = fix[xmd_bm::draw_index((float)u[k])];
-    float r = bm_radius(u[0], code[0]), sn, cs;
-    bm_angle(u[1], code[1], sn, cs);
-    out[0] = r * cs;
-    out[1] = r * sn;
-    float r3 = bm_radius(u[2], code[2]), sn3, cs3;
-    bm_angle(u[3], code[3], sn3, cs3);
-    out[2] = r3 * cs3;
+    const double l0 = bm_log_d(u[0]), l2 = bm_log_d(u[2]);
+    double s1, c1, s3, c3;
+    bm_sincos_d(u[1], s1, c1);
+    bm_sincos_d(u[3], s3, c3);
+    float L0 = (float)l0, L2 = (float)l2, S1 = (float)s1, C1 = (float)c1, C3 = (float)c3;
+    if (fx.tab) {
+        uint32_t i[4];
+        const float *t[4];
+        for (int k = 0; k < 4; ++k) {
+            i[k] = xmd_bm::draw_index(u[k]);
+            t[k] = fx.thr + 3 * (i[k] >> FIX_BUCKET_SHIFT);
+        }
+        const uint32_t k0 = near_boundary(l0, L0, t[0][0]) ? fx.tab[i[0]] : 0u;
+        const uint32_t k1 = near_boundary(s1, S1, t[1][1]) || near_boundary(c1, C1, t[1][2]) ? fx.tab[i[1]] : 0u;
+        const uint32_t k2 = near_boundary(l2, L2, t[2][0]) ? fx.tab[i[2]] : 0u;
+        const uint32_t k3 = near_boundary(c3, C3, t[3][2]) ? fx.tab[i[3]] : 0u;
+        L0 = xmd_bm::apply_fix(L0, k0 & 3u);
+        S1 = xmd_bm::apply_fix(S1, (k1 >> 2) & 3u);
+        C1 = xmd_bm::apply_fix(C1, (k1 >> 4) & 3u);
+        L2 = xmd_bm::apply_fix(L2, k2 & 3u);
+        C3 = xmd_bm::apply_fix(C3, (k3 >> 4) & 3u);
+    }
+    const float r = bm_radius(L0);
+    out[0] = r * C1;
+    out[1] = r * S1;
+    out[2] = bm_radius(L2) * C3;
 }
 
 }  // namespace xmd_dev
