@@ -203,6 +203,9 @@ def main():
     ap.add_argument("--long-steps", type=int, default=2000,
                     help="steps of the second timed window (0: none); the rate depends on how many list "
                          "rebuilds fall in a window, so it is reported beside the --steps one")
+    ap.add_argument("--force-lanes", type=int, default=0, choices=(0, 4, 8, 16),
+                    help="lanes per bead of the force kernel instead of the autotuner's choice (0: autotune); "
+                         "for counter runs, whose serialized kernels change the autotuner's timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy and a scale kernel of MIB MiB each "
                          "(known HBM bytes, to calibrate FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -226,31 +229,19 @@ def main():
     p_fp64 = synth.config_params(2, precision=_abi.PREC_FP64)
 
     # ---- batched mixed precision (the headline form)
+    if args.force_lanes:
+        p_mixed.force_lanes = args.force_lanes
     eng = Engine(p_mixed, batch)
     eng.step(args.warmup)
-    lanes, tune_ms = eng.autotune(reps=10)
-    log(f"autotune: lanes per bead {lanes} (ms per step for 4/8/16: {[round(x, 4) for x in tune_ms]})")
-    rb0 = eng.counters()["rebuilds"]
-    dt_mixed = timed_steps(eng, 0, args.steps)
-    value = batch.n_beads * args.steps / dt_mixed
-    rb_window = eng.counters()["rebuilds"] - rb0
-    log(f"mixed batched: {args.steps} steps in {dt_mixed:.3f}s -> {value:.3e} bead-steps/s "
-        f"({rb_window} list rebuilds over the replicas)")
-    # the rate depends on how many list rebuilds fall in the window (one per
-    # replica every ~70 steps here): beside the K-step figure, a window of at
-    # least 2000 steps
-    long_window = None
-    if args.long_steps > 0:
-        long_steps = max(args.long_steps, args.steps)
-        rb1 = eng.counters()["rebuilds"]
-        dt_long = timed_steps(eng, 0, long_steps)
-        long_window = {"steps": long_steps, "value": round(batch.n_beads * long_steps / dt_long, 1),
-                       "ms_per_step": round(1e3 * dt_long / long_steps, 4),
-                       "rebuilds": eng.counters()["rebuilds"] - rb1}
-        log(f"long window: {long_window}")
-    counters = eng.counters()
-
-    # ---- roofline of the dominant kernel, HIP events on the engine stream
+    if args.force_lanes:
+        lanes, tune_ms = args.force_lanes, None
+    else:
+        lanes, tune_ms = eng.autotune(reps=10)
+        log(f"autotune: lanes per bead {lanes} (ms per step for 4/8/16: {[round(x, 4) for x in tune_ms]})")
+    # ---- roofline of the dominant kernel, HIP events on the engine stream,
+    # over the steps right after the warm-up and the autotuner: the run is
+    # deterministic, so the kernel trace and counter commands of profiles/
+    # (same seeds, warm-up and profiled steps) see the same state
     prof = profile(eng, args.profile_steps)
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (launches, total_ms) = dom
@@ -270,6 +261,26 @@ def main():
     kernels = {k: {"launches": n, "avg_us": round(1e3 * ms / n, 2), "share": round(ms / sum(v[1] for v in prof.values()), 4)}
                for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     log("kernels:", json.dumps(kernels))
+
+    rb0 = eng.counters()["rebuilds"]
+    dt_mixed = timed_steps(eng, 0, args.steps)
+    value = batch.n_beads * args.steps / dt_mixed
+    rb_window = eng.counters()["rebuilds"] - rb0
+    log(f"mixed batched: {args.steps} steps in {dt_mixed:.3f}s -> {value:.3e} bead-steps/s "
+        f"({rb_window} list rebuilds over the replicas)")
+    # the rate depends on how many list rebuilds fall in the window (one per
+    # replica every ~70 steps here): beside the K-step figure, a window of at
+    # least 2000 steps
+    long_window = None
+    if args.long_steps > 0:
+        long_steps = max(args.long_steps, args.steps)
+        rb1 = eng.counters()["rebuilds"]
+        dt_long = timed_steps(eng, 0, long_steps)
+        long_window = {"steps": long_steps, "value": round(batch.n_beads * long_steps / dt_long, 1),
+                       "ms_per_step": round(1e3 * dt_long / long_steps, 4),
+                       "rebuilds": eng.counters()["rebuilds"] - rb1}
+        log(f"long window: {long_window}")
+    counters = eng.counters()
 
     # ---- whole-step algorithmic bytes per bead-step (integrator 348 B in the
     # mixed form; per pair term P_t (8 B index pair + 4 B parameter) + 48 B
@@ -389,7 +400,8 @@ def main():
         "rebuilds": counters["rebuilds"],
         "nl_entries": counters["nl_entries"],
         "qa_contacts": counters["qa_contacts"],
-        "autotune": {"lanes_per_bead": lanes, "ms_per_step_4_8_16": [round(x, 4) for x in tune_ms]},
+        "autotune": {"lanes_per_bead": lanes,
+                     "ms_per_step_4_8_16": [round(x, 4) for x in tune_ms] if tune_ms else "fixed by --force-lanes"},
         "secondary": extra,
         "roofline": roofline,
         "roofline_step": step_roof,
