@@ -209,6 +209,13 @@ typedef struct xmd_params {
        needed it alone back to the others, as for the rows.  No counterpart
        in freeform_params (qa.free_pairs is a vector, qa/free_pair.cpp). */
     int32_t fp_capacity;
+    /* capacity of each replica's QA contact set: 0 = automatic (16 per bead
+       plus 1024, every bead's full budget), n > 0 = n rounded up to 256.
+       Removed contacts leave vacancies; the set is compacted in order when
+       its extent passes half the capacity (results do not depend on it); a
+       set that outgrows the capacity halts the replica (XMD_ERR_CAPACITY).
+       No counterpart in freeform_params (qa.contacts is a set). */
+    int32_t ct_capacity;
 } xmd_params;
 
 /* System description.  All arrays are global over replicas, caller-owned and

@@ -61,6 +61,7 @@ CASES = {
     "groups": (lambda p, t: setattr(p, "stream_groups", -1), "stream_groups must be >= 0"),
     "row_width": (lambda p, t: setattr(p, "row_width", -5), "row_width must be >= 0"),
     "fp_capacity": (lambda p, t: setattr(p, "fp_capacity", -1), "fp_capacity must be >= 0"),
+    "ct_capacity": (lambda p, t: setattr(p, "ct_capacity", -1), "ct_capacity must be >= 0"),
     "no beads": (lambda p, t: setattr(t, "n_beads", 0), "empty topology"),
     "null atype": (lambda p, t: setattr(t, "atype", C.cast(None, C.POINTER(C.c_int8))), "null topology array"),
     "replica span": (lambda p, t: _int(t, "replica_offset", [0, 12, 23]), "replica_offset must span"),

@@ -72,3 +72,31 @@ def test_free_pairs_grow_like_a_larger_engine():
             assert np.array_equal(x, y), k
     a.close()
     b.close()
+
+
+def test_contact_compaction_keeps_results():
+    """QA contact sets of 512 entries, compacted in order whenever removed
+    contacts take their extent past 256 (the boxes hold up to ~240 live
+    contacts per replica by the end), against the automatic capacity that
+    never compacts here: after 3000 steps every replica's state, contact set
+    and rebuild count are equal bit for bit."""
+    system = synth.config_system(2, n_batch=3, chains=8, chain_len=60, horizon_steps=4000)
+    tight, auto = synth.config_params(2), synth.config_params(2)
+    tight.ct_capacity = 512
+    a, b = Engine(tight, system), Engine(auto, system)
+    for _ in range(6):
+        a.step(500)
+        b.step(500)
+    ca, cb = a.counters(), b.counters()
+    assert ca["steps"] == cb["steps"] == 3000 and ca["rebuilds"] == cb["rebuilds"]
+    n = 0
+    for k in range(system.n_replicas):
+        ya, _, ra, ta = a.get_state(k)
+        yb, _, rb, tb = b.get_state(k)
+        assert ra == rb and ta == tb, k
+        assert np.array_equal(ya, yb), k
+        n += contacts_equal(a, b, rep=k)
+    print("contacts at the end", n, flush=True)
+    assert n > 3 * 128  # above half the capacity on average: compaction ran
+    a.close()
+    b.close()

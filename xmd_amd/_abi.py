@@ -70,7 +70,7 @@ class XmdParams(C.Structure):
         ("vel_afm_H1", _d), ("vel_afm_H2", _d),
         ("heur_ang_enabled", _i32), ("heur_dih_enabled", _i32),
         ("heur_ang_coeffs", (_d * 7) * 9), ("heur_dih_coeffs", (_d * 6) * 9),
-        ("row_width", _i32), ("fp_capacity", _i32),
+        ("row_width", _i32), ("fp_capacity", _i32), ("ct_capacity", _i32),
     ]
 
 

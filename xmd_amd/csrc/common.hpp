@@ -93,7 +93,7 @@ struct Dev {
     uint8_t *nl_flag;
     // QA free pairs, candidates
     const int *fp_base, *fp_cap, *fp_chunk_rep;
-    int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt, *fp_slice_start;
+    int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt;
     int *fp_n;  // QA free pairs of each slot's row (set at rebuild)
     uint8_t *fp_vac;
     int *cand_cnt, *cand_fp, *cand_type;

@@ -312,16 +312,27 @@ struct Part {
     // one resident round of blocks of the grid-stride rebuild kernels (a
     // partial second round would leave most of the chip idle on a full rebuild)
     int grid_count = 0, grid_fill = 0;
+    bool fused_cells = false;  // every replica fits k_cells (CELLS_LDS cells, FUSED_MAX_BEADS beads)
     int nl_grid(bool fill = false) const { return std::min(nslot / NL_WAVES, fill ? grid_fill : grid_count); }
 
     // ------------------------------------------------------------ steps ---
-    void rebuild_seq(const double *Ppos, int p) {
-        int nb = nblk();
+    // Finalize (mode 1: a step's -- generator, time, verify decision; 0: the
+    // decision from force_rb only) and the cell lists of the replicas that
+    // rebuild: one launch when every replica fits k_cells.
+    void cells_seq(const double *Ppos, int p, int mode) {
+        if (fused_cells) {
+            launch("k_cells", [&] { k_cells<<<nrep, 1024, 0, stream>>>(D, G, Ppos, p, mode); });
+            return;
+        }
+        const int nb = nblk();
+        launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, mode); });
         launch("k_cell_count", [&] { k_cell_count<<<nb, BLK, 0, stream>>>(D, Ppos, p); });
-        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 0); });
+        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p); });
         launch("k_cell_fill", [&] { k_cell_fill<<<nb, BLK, 0, stream>>>(D, p); });
+    }
+    // Rows and QA free pairs of the replicas that rebuild.
+    void rows_seq(const double *Ppos, int p) {
         launch("k_nl_fill", [&] { k_nl_fill<<<nl_grid(true), BLK, 0, stream>>>(D, G, Ppos, p); });
-        launch("k_seg_scan", [&] { k_seg_scan<<<nrep, 1024, 0, stream>>>(D, p, 2); });
         launch("k_fp_write", [&] { k_fp_write<<<nl_grid(), BLK, 0, stream>>>(D, p); });
     }
     void step_seq(int p) {
@@ -333,8 +344,8 @@ struct Part {
             launch_force(P.precision == XMD_PREC_MIXED, false, lpb, nslot, stream, D, G, TTf, TTd, p, 1);
         });
         launch("k_integrate", [&] { k_integrate<<<nblk(), BLK, 0, stream>>>(D, G, p); });
-        launch("k_finalize", [&] { k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, p, 1); });
-        rebuild_seq(D.pos[1 - p], p);
+        cells_seq(D.pos[1 - p], p, 1);
+        rows_seq(D.pos[1 - p], p);
     }
     hipGraphExec_t capture(int p, int nsteps) {
         hipGraph_t g;
@@ -354,8 +365,8 @@ struct Part {
         std::vector<int> f(nrep);
         for (int k = 0; k < nrep; ++k) f[k] = pending_rb[k];
         put(D.force_rb, f);
-        k_finalize<<<nrep, WAVE, 0, stream>>>(D, G, cur, 0);
-        rebuild_seq(D.pos[cur], cur);
+        cells_seq(D.pos[cur], cur, 0);
+        rows_seq(D.pos[cur], cur);
         std::fill(pending_rb.begin(), pending_rb.end(), 0);
     }
     void ensure_lists() {
@@ -873,6 +884,9 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
         }
     }
     cell_off[E.nrep] = ncells;
+    E.fused_cells = true;
+    for (int k = 0; k < E.nrep; ++k)
+        if (cell_off[k + 1] - cell_off[k] > CELLS_LDS || E.rep_n[k] > FUSED_MAX_BEADS) E.fused_cells = false;
     // native contacts: rows of both ends, sorted
     std::vector<std::vector<std::pair<int, double>>> nrow(n);
     for (int c = 0; c < t->n_native; ++c) {
@@ -1119,14 +1133,15 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.nl_len = E.alloc<int>(n);
     D.fp_slice_cnt = E.alloc<int>(nsl);
     D.fp_n = E.alloc<int>(n);
-    D.fp_slice_start = E.alloc<int>(nsl);
     // QA contacts: at most MAXQ per bead -> 8n live; twice that for vacancies
     E.ct_base_h.resize(E.nrep + 1);
     E.ct_cap_h.resize(E.nrep + 1);
     int ct = 0;
     for (int k = 0; k < E.nrep; ++k) {
         E.ct_base_h[k] = ct;
-        E.ct_cap_h[k] = P.qa_enabled ? round_up(16 * E.rep_n[k] + 1024, BLK) : BLK;
+        E.ct_cap_h[k] = !P.qa_enabled       ? BLK
+                        : P.ct_capacity > 0 ? round_up(std::min(P.ct_capacity, 1 << 29), BLK)
+                                            : round_up(16 * E.rep_n[k] + 1024, BLK);
         ct += E.ct_cap_h[k];
     }
     E.ct_base_h[E.nrep] = ct;
@@ -1161,11 +1176,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.fp_base = E.upload(zero_base);
     D.fp_cap = E.upload(zero_base);
     D.fp_count = E.alloc<int>(E.nrep);
-    k_finalize<<<E.nrep, WAVE, 0, E.stream>>>(D, E.G, 0, 0);
-    int nb = E.nblk();
-    k_cell_count<<<nb, BLK, 0, E.stream>>>(D, D.pos[0], 0);
-    k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 0);
-    k_cell_fill<<<nb, BLK, 0, E.stream>>>(D, 0);
+    E.cells_seq(D.pos[0], 0, 0);
     k_nl_count<<<E.nl_grid(), BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
     HIPCHK(hipGetLastError());
     auto len = E.get(D.nl_len, n);
@@ -1229,9 +1240,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     if (P.qa_enabled && E.qa_lds() > 60 * 1024)
         throw Fail{XMD_ERR_CAPACITY, "too many replicas for one engine with QA enabled (at most 7680)"};
     // finish the first rebuild
-    k_nl_fill<<<E.nl_grid(true), BLK, 0, E.stream>>>(D, E.G, D.pos[0], 0);
-    k_seg_scan<<<E.nrep, 1024, 0, E.stream>>>(D, 0, 2);
-    k_fp_write<<<E.nl_grid(), BLK, 0, E.stream>>>(D, 0);
+    E.rows_seq(D.pos[0], 0);
     HIPCHK(hipGetLastError());
     std::fill(E.pending_rb.begin(), E.pending_rb.end(), 0);
     E.settle();  // a first list beyond a given fp_capacity grows it
@@ -1844,6 +1853,7 @@ void validate(const xmd_params *p, const xmd_topology *t) {
     if (p->stream_groups < 0) bad("stream_groups must be >= 0");
     if (p->row_width < 0) bad("row_width must be >= 0");
     if (p->fp_capacity < 0) bad("fp_capacity must be >= 0");
+    if (p->ct_capacity < 0) bad("ct_capacity must be >= 0");
     if (t->n_replicas <= 0 || t->n_beads <= 0 || !t->replica_offset || !t->chain_offset || !t->r0)
         bad("empty topology");
     if (!t->atype || !t->box || !t->temperature || !t->rng_state) bad("null topology array");

@@ -508,10 +508,10 @@ __global__ __launch_bounds__(BLK) void k_gather_force(Dev D) {
 
 // One wavefront per replica: advance rand_gen by 4N draws and the time by dt
 // (lang_pc.cpp:39-40), then decide the Verlet rebuild (verify.cpp:17-21).
-// mode 0: decision from force_rb only (no advance).
-__global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int mode) {
-    int rep = blockIdx.x;
-    int l = lane_id();
+// mode 0: decision from force_rb only (no advance).  Returns the decision
+// to every lane.
+__device__ inline int finalize_wave(const Dev &D, const Derived &G, int p, int mode, int rep) {
+    const int l = lane_id();
     if (rep == 0 && l == 0) D.any_rb[1 - p] = 0;
     if (mode == 1 && D.err[rep]) {
         // halted replica: generator and time carried over unchanged, no
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
             D.maxd[1 - p][rep] = 0ull;
             D.rb_now[rep] = 0;
         }
-        return;
+        return 0;
     }
     if (mode == 1) {
         uint64_t Sn = wave_matvec_rows(D.jadv + 64 * rep, D.rng[p][rep]);
@@ -532,8 +532,9 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
             D.n_done[rep] += 1;
         }
     }
+    int rb = 0;
     if (l == 0) {
-        int rb = D.force_rb[rep];
+        rb = D.force_rb[rep];
         if (mode == 1) {
             double m = xmd_bm::bitsd(D.maxd[p][rep]);
             if (!(m == m)) atomicOr(D.err + rep, E_NONFINITE);
@@ -548,6 +549,10 @@ __global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int 
         D.force_rb[rep] = 0;
         if (rb) atomicOr(D.any_rb + p, 1);
     }
+    return __shfl(rb, 0);
+}
+__global__ __launch_bounds__(WAVE) void k_finalize(Dev D, Derived G, int p, int mode) {
+    finalize_wave(D, G, p, mode, blockIdx.x);
 }
 
 // ------------------------------------------------------------- rebuild ---
@@ -557,11 +562,88 @@ __device__ inline int cell_coord(double x, double L, int nc) {
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
+// One workgroup per replica: finalize_wave, then -- when the replica
+// rebuilds -- its whole cell list: counts in LDS, exclusive scan, scatter
+// (cell_start / cell_end, cell_beads, bead_cell), and the free-pair slice
+// counters the row fill adds to set to zero.  The engine uses it when every
+// replica has at most CELLS_LDS cells and FUSED_MAX_BEADS beads, so that an
+// idle step's finalize and cell phase is one small launch; larger replicas
+// take k_finalize, k_cell_count, k_seg_scan and k_cell_fill.
+constexpr int CELLS_LDS = 15360;
+constexpr int FUSED_MAX_BEADS = 32768;
+__global__ __launch_bounds__(1024) void k_cells(Dev D, Derived G, const double *P, int p, int mode) {
+    __shared__ int s_c[CELLS_LDS];
+    __shared__ int s_wsum[1024 / WAVE];
+    __shared__ int s_rb;
+    const int rep = blockIdx.x, tid = threadIdx.x, wid = tid / WAVE, l = lane_id();
+    if (wid == 0) {
+        const int rb = finalize_wave(D, G, p, mode, rep);
+        if (l == 0) s_rb = rb;
+    }
+    __syncthreads();
+    if (!s_rb) return;
+    const int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
+    for (int sl = (a >> 6) + tid; sl < ((b + WAVE - 1) >> 6); sl += 1024) D.fp_slice_cnt[sl] = 0;
+    const int c0 = D.cell_off[rep], nc = D.cell_off[rep + 1] - c0;
+    if (nc == 0) return;  // open box: the row fill scans the replica
+    const int *nd = D.cell_dim + 3 * rep;
+    const Box box = D.box[rep];
+    for (int c = tid; c < nc; c += 1024) s_c[c] = 0;
+    __syncthreads();
+    for (int s = a + tid; s < b; s += 1024) {
+        if (!(D.bflags[s] & B_ACTIVE)) continue;
+        const v3<double> r = ldp(P, s);
+        const int cx = cell_coord(r.x, box.cx, nd[0]), cy = cell_coord(r.y, box.cy, nd[1]),
+                  cz = cell_coord(r.z, box.cz, nd[2]);
+        const int c = (cz * nd[1] + cy) * nd[0] + cx;
+        D.bead_cell[s] = c0 + c;
+        atomicAdd(&s_c[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan in chunks of 1024 cells; cell_start is the first
+    // entry of the cell in cell_beads (absolute: the replica's beads start
+    // at its first slot)
+    int carry = 0;
+    for (int c00 = 0; c00 < nc; c00 += 1024) {
+        const int c = c00 + tid;
+        const int v = c < nc ? s_c[c] : 0;
+        int x = v;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (l >= o) x += y;
+        }
+        if (l == WAVE - 1) s_wsum[wid] = x;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int k = 0; k < 1024 / WAVE; ++k) {
+            const int t = s_wsum[k];
+            if (k < wid) woff += t;
+            tot += t;
+        }
+        if (c < nc) {
+            const int ex = carry + woff + x - v;
+            s_c[c] = ex;
+            D.cell_start[c0 + c] = a + ex;
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    for (int s = a + tid; s < b; s += 1024) {
+        if (!(D.bflags[s] & B_ACTIVE)) continue;
+        const int k = atomicAdd(&s_c[D.bead_cell[s] - c0], 1);
+        D.cell_beads[a + k] = s;
+    }
+    __syncthreads();
+    for (int c = tid; c < nc; c += 1024) D.cell_end[c0 + c] = a + s_c[c];
+}
+
 __global__ __launch_bounds__(BLK) void k_cell_count(Dev D, const double *P, int p) {
     if (!D.any_rb[p]) return;
     int s = blockIdx.x * BLK + threadIdx.x;
     int rep = D.slot_rep[s];
-    if (!(D.bflags[s] & B_ACTIVE) || !D.rb_now[rep]) return;
+    if (!D.rb_now[rep]) return;
+    if ((s & (WAVE - 1)) == 0) D.fp_slice_cnt[s >> 6] = 0;  // the row fill adds to it
+    if (!(D.bflags[s] & B_ACTIVE)) return;
     int c0 = D.cell_off[rep];
     if (D.cell_off[rep + 1] == c0) return;
     const int *nd = D.cell_dim + 3 * rep;
@@ -573,33 +655,20 @@ __global__ __launch_bounds__(BLK) void k_cell_count(Dev D, const double *P, int 
     atomicAdd(D.cell_cnt + c, 1);
 }
 
-// One workgroup per replica: exclusive scan of a per-replica segment.
-// mode 0: cells (cell_cnt -> cell_start/end/cursor, absolute slot offsets);
-// mode 2: free-pair slices (fp_slice_cnt -> fp_slice_start, fp_count).
-__global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
+// One workgroup per replica: exclusive scan of the replica's cell counts
+// (cell_cnt -> cell_start / cell_end / cell_cursor, absolute slot offsets).
+__global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p) {
     __shared__ int s_buf[1024];
     __shared__ int s_carry;
     if (!D.any_rb[p]) return;
     int rep = blockIdx.x, tid = threadIdx.x;
     if (!D.rb_now[rep]) return;
-    int lo, hi, base;
-    if (mode == 0) {
-        lo = D.cell_off[rep];
-        hi = D.cell_off[rep + 1];
-        base = D.rep_slot_off[rep];
-    } else {
-        lo = D.rep_slot_off[rep] >> 6;
-        hi = (D.rep_slot_off[rep] + D.rep_n[rep] + 63) >> 6;
-        base = D.fp_base[rep];
-    }
+    const int lo = D.cell_off[rep], hi = D.cell_off[rep + 1], base = D.rep_slot_off[rep];
     if (tid == 0) s_carry = base;
     __syncthreads();
     for (int c0 = lo; c0 < hi; c0 += 1024) {
         int c = c0 + tid;
-        int v = 0;
-        if (c < hi) {
-            v = mode == 0 ? D.cell_cnt[c] : D.fp_slice_cnt[c];
-        }
+        int v = c < hi ? D.cell_cnt[c] : 0;
         s_buf[tid] = v;
         __syncthreads();
         for (int sh = 1; sh < 1024; sh <<= 1) {
@@ -610,29 +679,13 @@ __global__ __launch_bounds__(1024) void k_seg_scan(Dev D, int p, int mode) {
         }
         int ex = s_carry + s_buf[tid] - v;
         if (c < hi) {
-            if (mode == 0) {
-                D.cell_start[c] = ex;
-                D.cell_end[c] = ex + v;
-                D.cell_cursor[c] = ex;
-            } else {
-                D.fp_slice_start[c] = ex;
-            }
+            D.cell_start[c] = ex;
+            D.cell_end[c] = ex + v;
+            D.cell_cursor[c] = ex;
         }
         __syncthreads();
         if (tid == 1023) s_carry = ex + v;
         __syncthreads();
-    }
-    if (tid == 0) {
-        int used = s_carry - base;
-        if (mode == 2) {
-            if (used > D.fp_cap[rep]) {
-                // the count the host grows the replica's region to (the
-                // overflow of the QA pass leaves it 0: that one halts)
-                atomicOr(D.err + rep, E_FP_CAP);
-                D.need_fp[rep] = used;
-            }
-            D.fp_count[rep] = used > D.fp_cap[rep] ? D.fp_cap[rep] : used;
-        }
     }
 }
 
@@ -854,7 +907,10 @@ __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double 
 
 // Free pairs in canonical order (slot, then row order); zero the cell
 // counters; count the rebuild.  Wavefront per slot: the slot's pairs follow
-// those of the lower slots of its 64-slot slice.
+// those of the replica's lower slices (their counts, added by the row fill,
+// summed here) and of the lower slots of its own 64-slot slice.  The wave of
+// the first slot of the replica's last slice sets the replica's count and
+// flags a capacity overflow (with the count it needed).
 __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
     if (!D.any_rb[p]) return;
     const int l = lane_id();
@@ -862,10 +918,15 @@ __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
         const int s = s0 + threadIdx.x / WAVE;
         const int rep = D.slot_rep[s];
         if (!D.rb_now[rep]) continue;
-        const int sl = s >> 6;
+        const int sl = s >> 6, sl0 = D.rep_slot_off[rep] >> 6;
+        int pre = 0;
+        for (int t = sl0 + l; t < sl; t += WAVE) pre += D.fp_slice_cnt[t];
         int before = l < (s & 63) ? D.fp_n[(sl << 6) + l] : 0;
-        for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o);
-        int off = D.fp_slice_start[sl] + before;
+        for (int o = 32; o > 0; o >>= 1) {
+            pre += __shfl_xor(pre, o);
+            before += __shfl_xor(before, o);
+        }
+        int off = D.fp_base[rep] + pre + before;
         const int end = D.fp_base[rep] + D.fp_cap[rep];
         const int base = D.slice_start[sl] + (s & 63);
         const int len = (D.bflags[s] & B_ACTIVE) && D.fp_n[s] ? D.nl_len[s] : 0;
@@ -882,11 +943,20 @@ __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
             off += __popcll(m);
         }
         if (l == 0) {
-            if ((s & 63) == 0) D.fp_slice_cnt[sl] = 0;  // consumed by k_seg_scan; zero for the next rebuild
             const int local = s - D.rep_slot_off[rep];
             const int c0 = D.cell_off[rep], nc = D.cell_off[rep + 1] - c0;
             for (int c = local; c < nc; c += D.rep_n[rep]) D.cell_cnt[c0 + c] = 0;
             if (local == 0) D.n_rebuild[rep] += 1;
+            if ((s & 63) == 0 && sl == ((D.rep_slot_off[rep] + D.rep_n[rep] - 1) >> 6)) {
+                const int used = pre + D.fp_slice_cnt[sl];
+                if (used > D.fp_cap[rep]) {
+                    // the count the host grows the replica's region to (the
+                    // overflow of the QA pass leaves it 0: that one halts)
+                    atomicOr(D.err + rep, E_FP_CAP);
+                    D.need_fp[rep] = used;
+                }
+                D.fp_count[rep] = used > D.fp_cap[rep] ? D.fp_cap[rep] : used;
+            }
         }
     }
 }
