@@ -42,6 +42,27 @@ def timed_steps(eng, warmup, steps):
     return time.perf_counter() - t0
 
 
+def tune_groups(params, system, candidates=(1, 2, 4), warm=150, steps=150):
+    """Stream-group count of the batched engine (fixed at creation, so
+    chosen by creating one engine per candidate): each warms up, autotunes
+    its force tile and times `steps` steps from the same state; returns the
+    count to use -- the engine's default (2) unless another is more than 3 %
+    faster -- and the ms per step of each.  Per-replica results do not
+    depend on the count."""
+    ms = []
+    for g in candidates:
+        params.stream_groups = g
+        e = Engine(params, system)
+        e.step(warm)
+        e.autotune(reps=5)
+        ms.append(1e3 * timed_steps(e, 0, steps) / steps)
+        e.close()
+    params.stream_groups = 0
+    best, default = int(np.argmin(ms)), candidates.index(2)
+    pick = candidates[best] if ms[best] < 0.97 * ms[default] else 2
+    return pick, [round(x, 4) for x in ms]
+
+
 def row_entries(eng):
     return eng.counters()["nl_entries"]
 
@@ -203,6 +224,8 @@ def main():
     ap.add_argument("--long-steps", type=int, default=2000,
                     help="steps of the second timed window (0: none); the rate depends on how many list "
                          "rebuilds fall in a window, so it is reported beside the --steps one")
+    ap.add_argument("--stream-groups", type=int, default=0,
+                    help="stream groups of the batched engine instead of the sweep over 1, 2, 4 (0: sweep)")
     ap.add_argument("--force-lanes", type=int, default=0, choices=(0, 4, 8, 16),
                     help="lanes per bead of the force kernel instead of the autotuner's choice (0: autotune); "
                          "for counter runs, whose serialized kernels change the autotuner's timings")
@@ -231,6 +254,14 @@ def main():
     # ---- batched mixed precision (the headline form)
     if args.force_lanes:
         p_mixed.force_lanes = args.force_lanes
+    groups, groups_ms = None, None
+    if args.stream_groups:
+        p_mixed.stream_groups = args.stream_groups
+        groups = args.stream_groups
+    elif args.replicas > 1:
+        groups, groups_ms = tune_groups(p_mixed, batch)
+        p_mixed.stream_groups = groups
+        log(f"stream groups: {groups} (ms per step for 1/2/4: {groups_ms})")
     eng = Engine(p_mixed, batch)
     eng.step(args.warmup)
     if args.force_lanes:
@@ -397,9 +428,17 @@ def main():
         "force_rel_rms_mixed_vs_fp64_per_term": rel_terms,
         "long_window": long_window,
         "window_rebuilds": rb_window,
+        # a window shorter than a few rebuild intervals per replica times a
+        # rate that depends on whether rebuilds fall inside it: flagged, with
+        # the long window beside it
+        "window_note": (None if rb_window >= 3 * args.replicas else
+                        f"timed window of {args.steps} steps holds {rb_window} list rebuilds over "
+                        f"{args.replicas} replicas (fewer than 3 per replica): its rate is not representative; "
+                        f"see long_window"),
         "rebuilds": counters["rebuilds"],
         "nl_entries": counters["nl_entries"],
         "qa_contacts": counters["qa_contacts"],
+        "stream_groups": {"chosen": groups, "ms_per_step_1_2_4": groups_ms if groups_ms else "fixed by --stream-groups"},
         "autotune": {"lanes_per_bead": lanes,
                      "ms_per_step_4_8_16": [round(x, 4) for x in tune_ms] if tune_ms else "fixed by --force-lanes"},
         "secondary": extra,
