@@ -208,3 +208,33 @@ def test_mixed_precision_full_size(cid):
     assert np.allclose(Vmx.sum(axis=1), V64.sum(axis=1), rtol=1e-5, atol=1e-6)
     e64.close()
     emx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_c1_runs_its_step_count(prec):
+    """C1 (one 1000-bead chain, open box) stepped for its stated 100,000
+    steps (a seed whose noise has no zero draw in that horizon): every
+    replica step is taken, the state stays finite, the chain keeps its
+    bonds (every tether within 20 % of its native length) and the kinetic
+    energy per bead stays at the bath's scale -- 3/2 <v^2> with the
+    stationary <v^2> = 1.22 T dt per component of the reference's noise,
+    which lacks the dt^-1/2 factor (lang_pc.cpp:15-19)."""
+    system = synth.config_system(1, horizon_steps=100000)
+    p = synth.config_params(1, precision={"fp64": _abi.PREC_FP64, "mixed": _abi.PREC_MIXED}[prec])
+    eng = Engine(p, system)
+    ks = []
+    for _ in range(10):
+        eng.step(10000)
+        obs = eng.observables()[0]  # t, V, K, E, Rg, W, active natives, natives
+        ks.append(obs[2] / system.n_beads)
+    c = eng.counters()
+    y, _, _, t = eng.get_state(0)
+    print(prec, "t", t, "K per bead", [round(k, 4) for k in ks], "rebuilds", c["rebuilds"], flush=True)
+    assert c["steps"] == 100000 and eng.replica_status(0)[0] == 0
+    assert np.isfinite(y).all()
+    r = y[0]
+    bond = np.linalg.norm(r[:, 1:] - r[:, :-1], axis=0)
+    assert np.abs(bond / (synth.BOND * synth.ANGSTROM) - 1).max() < 0.2
+    k_bath = 1.5 * 1.22 * float(system.temperature[0]) * p.dt
+    assert 0.3 * k_bath < np.mean(ks) < 3.0 * k_bath, (np.mean(ks), k_bath)
+    eng.close()
