@@ -102,7 +102,9 @@ def test_oracle_follows_engine_into_blowup():
             gap = np.abs(ro - e.positions(0)).max()
             if s == S0 + 10:
                 assert gap < 1e-10, gap  # the hand-over is exact up to rounding
-            assert gap < 1e-2, (s, gap)  # ... and stays within the chaotic drift
+            # ... and stays within the chaotic drift (a tenth of an angstrom,
+            # against 3.8 A bonds: the same trajectory, not another one)
+            assert gap < 0.1, (s, gap)
     e.close()
     ora.close()
     print("oracle non-finite at step", ora_bad, flush=True)

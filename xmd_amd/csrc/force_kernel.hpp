@@ -136,6 +136,8 @@ __device__ inline void angle_bond_task(const Dev &D, const Derived &G, const Ter
     L.tV[a] = Vb;
 }
 
+constexpr int SS_TYPES = 400;  // residue pairs 20 a1 + a2
+
 // Whole wavefronts per task kind (quads from wave 0, angles and bonds from
 // the next free wave) so that no wavefront runs both code paths.
 template <typename T, int LPB>
@@ -159,8 +161,21 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     const int s = t / LPB, sub = t % LPB;
     const int s0 = blk * (BLK / LPB);  // the block's first bead
     __shared__ BondedLds<LPB> L;
-    bonded_phase<T, LPB>(D, G, TT.P, TT.mask, D.pos[p], s0, L);
-    const int rep = D.slot_rep[s];
+    // the PID bundle's ss LJ table (depth, r_min, r_max by residue pair;
+    // float copies in the FP32 form), read per bundle from LDS
+    __shared__ T ss_tab[3 * SS_TYPES];
+    {
+        const bool f32 = sizeof(T) == 4;
+        for (int q = threadIdx.x; q < SS_TYPES; q += BLK) {
+            ss_tab[q] = f32 ? (T)D.ss_depth_f[q] : (T)D.ss_depth[q];
+            ss_tab[SS_TYPES + q] = f32 ? (T)D.ss_rmin_f[q] : (T)D.ss_rmin[q];
+            ss_tab[2 * SS_TYPES + q] = f32 ? (T)D.ss_rmax_f[q] : (T)D.ss_rmax[q];
+        }
+    }
+    bonded_phase<T, LPB>(D, G, TT.P, TT.mask, D.pos[p], s0, L);  // ends with a barrier
+    // a wavefront's beads lie in one 64-aligned slot range, hence one
+    // replica: its index (and so the box) is wave-uniform, in scalar registers
+    const int rep = __builtin_amdgcn_readfirstlane(D.slot_rep[s]);
     const uint8_t bf = D.bflags[s];
     // a halted replica (error word set) is skipped: nothing it computes
     // would be used, and its wall pair states must not change
@@ -234,11 +249,20 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
             const T q_s = (T)D.charge[D.atype[s]];
             const v3<double> rsp = (bf & B_PREV) ? ldp(Pp, s - 1) : ri;
             const v3<double> rsn = (bf & B_NEXT) ? ldp(Pp, s + 1) : ri;
-            for (int k = sub; k < len; k += LPB) {
-                const int e = base + 64 * k;
-                const uint8_t fl = D.nl_flag[e];
+            // each lane loads its next entry's flags and partner one
+            // iteration ahead, so an iteration waits on one round of
+            // position gathers only
+            int e = base + 64 * sub;
+            uint8_t fl_n = sub < len ? D.nl_flag[e] : 0;
+            int j_n = sub < len ? D.nl_j[e] : 0;
+            for (int k = sub; k < len; k += LPB, e += 64 * LPB) {
+                const uint8_t fl = fl_n;
+                const int j = j_n;
+                if (k + LPB < len) {
+                    fl_n = D.nl_flag[e + 64 * LPB];
+                    j_n = D.nl_j[e + 64 * LPB];
+                }
                 if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
-                const int j = D.nl_j[e];
                 const v3<double> rj = ldp(Pp, j);
                 const v3<double> dd = wrap(box, rj - ri);
                 const v3<T> d = cvt<T>(dd);
@@ -272,11 +296,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                     const int i1 = owner ? s : j, i2 = owner ? j : s;
                     const int ty = 20 * D.atype[i1] + D.atype[i2];
                     v3<T> out[6];
-                    const bool f32 = sizeof(T) == 4;  // float copies of the ss tables in the FP32 form
-                    const T ssd = f32 ? (T)D.ss_depth_f[ty] : (T)D.ss_depth[ty];
-                    const T ssn = f32 ? (T)D.ss_rmin_f[ty] : (T)D.ss_rmin[ty];
-                    const T ssx = f32 ? (T)D.ss_rmax_f[ty] : (T)D.ss_rmax[ty];
-                    T v = pid_bundle<T>(P, ssd, ssn, ssx,
+                    T v = pid_bundle<T>(P, ss_tab[ty], ss_tab[SS_TYPES + ty], ss_tab[2 * SS_TYPES + ty],
                                         pid_side<T>(ri - rsp, rsn - ri, rsn - rsp),
                                         pid_side<T>(rj - rjp, rjn - rj, rjn - rjp), dd, out);
                     add3(Fx, Fy, Fz, out[1]);
