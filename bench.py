@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 from xmd_amd import _abi, synth  # noqa: E402
 from xmd_amd.engine import Engine, profile  # noqa: E402
+from xmd_amd._host import XmdError  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -35,11 +36,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_steps(eng, warmup, steps):
-    eng.step(warmup)  # xmd_step synchronizes before returning
+NONFINITE = 4  # XMD_ERR_NONFINITE: a replica's positions (or its noise) turned non-finite
+
+
+def step_tolerant(eng, steps):
+    """eng.step(steps), letting replicas halt: a replica whose positions turn
+    non-finite (the model's own dynamics, or a zero noise draw) stops where
+    it failed while the others take every step (the engine's per-replica
+    errors); any other error is raised."""
+    try:
+        eng.step(steps)  # xmd_step synchronizes before returning
+    except XmdError as err:
+        if err.status != NONFINITE:
+            raise
+        log(f"halted: {err}")
+
+
+def running_beads(eng, system):
+    """Beads of the replicas that are not halted."""
+    return sum(system.replica_offset[k + 1] - system.replica_offset[k] for k in range(system.n_replicas)
+               if eng.replica_status(k)[0] == 0)
+
+
+def timed_steps(eng, warmup, steps, system=None):
+    """Seconds for `steps` steps after `warmup`; with `system`, also the
+    beads that count for the window: those of the replicas still running at
+    its end (a replica that halts inside the window counts for none of it)."""
+    step_tolerant(eng, warmup)
     t0 = time.perf_counter()
-    eng.step(steps)
-    return time.perf_counter() - t0
+    step_tolerant(eng, steps)
+    dt = time.perf_counter() - t0
+    return dt if system is None else (dt, running_beads(eng, system))
 
 
 def tune_groups(params, system, candidates=(1, 2, 4), warm=150, steps=150):
@@ -53,7 +80,7 @@ def tune_groups(params, system, candidates=(1, 2, 4), warm=150, steps=150):
     for g in candidates:
         params.stream_groups = g
         e = Engine(params, system)
-        e.step(warm)
+        step_tolerant(e, warm)
         e.autotune(reps=5)
         ms.append(1e3 * timed_steps(e, 0, steps) / steps)
         e.close()
@@ -263,7 +290,7 @@ def main():
         p_mixed.stream_groups = groups
         log(f"stream groups: {groups} (ms per step for 1/2/4: {groups_ms})")
     eng = Engine(p_mixed, batch)
-    eng.step(args.warmup)
+    step_tolerant(eng, args.warmup)
     if args.force_lanes:
         lanes, tune_ms = args.force_lanes, None
     else:
@@ -273,7 +300,13 @@ def main():
     # over the steps right after the warm-up and the autotuner: the run is
     # deterministic, so the kernel trace and counter commands of profiles/
     # (same seeds, warm-up and profiled steps) see the same state
-    prof = profile(eng, args.profile_steps)
+    try:
+        prof = profile(eng, args.profile_steps)
+    except XmdError as err:
+        if err.status != NONFINITE:
+            raise
+        log(f"halted while profiling: {err}; profiling again")
+        prof = profile(eng, args.profile_steps)
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (launches, total_ms) = dom
     avg_ms = total_ms / launches
@@ -294,8 +327,8 @@ def main():
     log("kernels:", json.dumps(kernels))
 
     rb0 = eng.counters()["rebuilds"]
-    dt_mixed = timed_steps(eng, 0, args.steps)
-    value = batch.n_beads * args.steps / dt_mixed
+    dt_mixed, beads_mixed = timed_steps(eng, 0, args.steps, batch)
+    value = beads_mixed * args.steps / dt_mixed
     rb_window = eng.counters()["rebuilds"] - rb0
     log(f"mixed batched: {args.steps} steps in {dt_mixed:.3f}s -> {value:.3e} bead-steps/s "
         f"({rb_window} list rebuilds over the replicas)")
@@ -306,8 +339,8 @@ def main():
     if args.long_steps > 0:
         long_steps = max(args.long_steps, args.steps)
         rb1 = eng.counters()["rebuilds"]
-        dt_long = timed_steps(eng, 0, long_steps)
-        long_window = {"steps": long_steps, "value": round(batch.n_beads * long_steps / dt_long, 1),
+        dt_long, beads_long = timed_steps(eng, 0, long_steps, batch)
+        long_window = {"steps": long_steps, "value": round(beads_long * long_steps / dt_long, 1),
                        "ms_per_step": round(1e3 * dt_long / long_steps, 4),
                        "rebuilds": eng.counters()["rebuilds"] - rb1}
         log(f"long window: {long_window}")
@@ -342,8 +375,8 @@ def main():
     if not args.no_fp64:
         eb = Engine(p_fp64, batch)
         kb = max(100, args.steps // 10)
-        dtb = timed_steps(eb, min(args.warmup, 50), kb)
-        fp64_batched = batch.n_beads * kb / dtb
+        dtb, beads_b = timed_steps(eb, min(args.warmup, 50), kb, batch)
+        fp64_batched = beads_b * kb / dtb
         log(f"fp64 batched: {kb} steps in {dtb:.3f}s -> {fp64_batched:.3e} bead-steps/s")
         eb.close()
         e64 = Engine(p_fp64, single)
@@ -387,10 +420,10 @@ def main():
                 if cid == 4:
                     c4 = sysx
             ex = Engine(synth.config_params(cid, precision=_abi.PREC_MIXED), sysx)
-            ex.step(100)
+            step_tolerant(ex, 100)
             lx, _ = ex.autotune(reps=5)
-            dtx = timed_steps(ex, 0, args.extra_steps)
-            extra[key] = {"value": round(sysx.n_beads * args.extra_steps / dtx, 1), "unit": "bead-steps/s",
+            dtx, beads_x = timed_steps(ex, 0, args.extra_steps, sysx)
+            extra[key] = {"value": round(beads_x * args.extra_steps / dtx, 1), "unit": "bead-steps/s",
                           "beads": int(sysx.n_beads), "replicas": int(sysx.n_replicas), "steps": args.extra_steps,
                           "ms_per_step": round(1e3 * dtx / args.extra_steps, 4), "lanes_per_bead": lx}
             log(f"{key}: {extra[key]} (setup {time.time() - t1:.1f}s)")
@@ -435,6 +468,7 @@ def main():
                         f"timed window of {args.steps} steps holds {rb_window} list rebuilds over "
                         f"{args.replicas} replicas (fewer than 3 per replica): its rate is not representative; "
                         f"see long_window"),
+        "halted_replicas": [k for k in range(batch.n_replicas) if eng.replica_status(k)[0] != 0],
         "rebuilds": counters["rebuilds"],
         "nl_entries": counters["nl_entries"],
         "qa_contacts": counters["qa_contacts"],

@@ -403,9 +403,11 @@ xmd_status xmd_observables_noalloc(xmd_engine *eng, double active_threshold, dou
  * force kernel with 4, 8 and 16 lanes per bead on the current state, every
  * stream group's launches side by side as in a step, in three interleaved
  * rounds; keeps the candidate with the lowest median for every later step
- * and returns it in *lpb_out and the three medians (ms per step: the span of
- * one launch of every group) in ms_out[3] (either may be NULL).  The
- * simulation state is not changed. */
+ * (8, the default, unless the lowest is more than 3 % below its median: the
+ * tiles sum a bead's partial forces in different orders, so trajectories
+ * differ in rounding between tiles) and returns it in *lpb_out and the three
+ * medians (ms per step: the span of one launch of every group) in ms_out[3]
+ * (either may be NULL).  The simulation state is not changed. */
 xmd_status xmd_autotune(xmd_engine *eng, int32_t reps, int32_t *lpb_out, double *ms_out);
 #ifdef __cplusplus
 }

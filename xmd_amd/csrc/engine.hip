@@ -2266,17 +2266,22 @@ xmd_status xmd_autotune(xmd_engine *e, int32_t reps, int32_t *lpb_out, double *m
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
     if (st != XMD_OK) return st;
-    double best = 0.0;
+    // the fastest tile, unless the default is within 3 % of it: tiles sum
+    // a bead's partial forces in different orders, so a choice that follows
+    // timing noise would make the trajectory differ from run to run
+    double best = 0.0, dflt = 0.0;
     int pick = FORCE_LPB;
     for (int c = 0; c < 3; ++c) {
         std::sort(t[c].begin(), t[c].end());
         const double med = t[c][ROUNDS / 2];
         if (ms_out) ms_out[c] = med;
+        if (cand[c] == FORCE_LPB) dflt = med;
         if (c == 0 || med < best) {
             best = med;
             pick = cand[c];
         }
     }
+    if (dflt <= 1.03 * best) pick = FORCE_LPB;
     for (Part *q : e->parts) {
         st = on_part(e, q, run_guarded(q->err, [&]() { q->set_lpb(pick); }));
         if (st != XMD_OK) return st;
