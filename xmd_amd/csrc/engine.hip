@@ -312,6 +312,11 @@ struct Part {
     // one resident round of blocks of the grid-stride rebuild kernels (a
     // partial second round would leave most of the chip idle on a full rebuild)
     int grid_count = 0, grid_fill = 0;
+    // the QA pass: one resident round of blocks striding over the live
+    // chunks (the chunk capacity is about twice what is live, and idle
+    // blocks still cost their dispatch and prefix scan)
+    int grid_qa = 0;
+    int qa_grid_for(int chunks) const { return std::max(1, std::min(chunks, grid_qa)); }
     bool fused_cells = false;  // every replica fits k_cells (CELLS_LDS cells, FUSED_MAX_BEADS beads)
     int nl_grid(bool fill = false) const { return std::min(nslot / NL_WAVES, fill ? grid_fill : grid_count); }
 
@@ -473,7 +478,7 @@ struct Part {
         put(const_cast<int *>(D.fp_cap), capv);
         fp_total = (int)fp;
         nfp_chunks = (int)(fp / BLK);
-        qa_grid = std::max(1, std::min(nfp_chunks + nct_chunks, 8192));
+        qa_grid = qa_grid_for(nfp_chunks + nct_chunks);
         drop_graphs();  // they hold the old Dev
     }
     void widen_rows(int W2) {
@@ -781,13 +786,16 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Fail{XMD_ERR_NO_DEVICE, "no HIP device"};
     HIPCHK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
     {
-        int dev = 0, ncu = 0, bc = 0, bf = 0;
+        int dev = 0, ncu = 0, bc = 0, bf = 0, bq = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bc, k_nl_count, BLK, 0));
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bf, k_nl_fill, BLK, 0));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bq, k_qa_pairs, BLK,
+                                                            (2 * (size_t)std::max(t->n_replicas, 1) + 1) * sizeof(int)));
         E.grid_count = std::max(1, bc) * std::max(1, ncu);
         E.grid_fill = std::max(1, bf) * std::max(1, ncu);
+        E.grid_qa = std::max(1, bq) * std::max(1, ncu);
     }
     E.nrep = t->n_replicas;
     E.nbeads = t->n_beads;
@@ -1236,7 +1244,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
         for (int c = E.fp_base_h[k] / BLK; c < (E.fp_base_h[k] + E.fp_cap_h[k]) / BLK; ++c) fprep[c] = k;
     D.fp_chunk_rep = E.upload(fprep);
     E.nfp_chunks = (int)(fp / BLK);
-    E.qa_grid = std::max(1, std::min(E.nfp_chunks + E.nct_chunks, 8192));
+    E.qa_grid = E.qa_grid_for(E.nfp_chunks + E.nct_chunks);
     if (P.qa_enabled && E.qa_lds() > 60 * 1024)
         throw Fail{XMD_ERR_CAPACITY, "too many replicas for one engine with QA enabled (at most 7680)"};
     // finish the first rebuild
