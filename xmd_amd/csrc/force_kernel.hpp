@@ -48,8 +48,12 @@ template <int LPB> __device__ inline double lane_sum(double v) {
 // bits).  Forces per bead position and energies per term, in FP64.
 constexpr int QUAD_TERMS[4] = {T_CHIRAL, T_CND, T_SND, T_HEUR_DIH};
 constexpr int ANG_TERMS[2] = {T_NAT_ANG, T_HEUR_ANG};
+// The force kernel's workgroup: two wavefronts, one for the block's quads
+// and one for its angles and bonds, so that no wavefront of the block waits
+// idle for the bonded phase.
+constexpr int FBLK = 128;
 template <int LPB> struct BondedLds {
-    static constexpr int NB = BLK / LPB, NQ = NB + 3, NA = NB + 2;
+    static constexpr int NB = FBLK / LPB, NQ = NB + 3, NA = NB + 2;
     v3<double> qF[NQ][4];
     double qV[NQ][4];
     v3<double> aF[NA][3];
@@ -145,7 +149,7 @@ __device__ inline void bonded_phase(const Dev &D, const Derived &G, const TermPa
                                     const double *Pp, int s0, BondedLds<LPB> &L) {
     using BL = BondedLds<LPB>;
     constexpr int A0 = (BL::NQ + WAVE - 1) / WAVE * WAVE;
-    for (int task = threadIdx.x; task < A0 + BL::NA; task += BLK) {
+    for (int task = threadIdx.x; task < A0 + BL::NA; task += FBLK) {
         if (task < BL::NQ) quad_task<T, LPB>(D, P, mask, Pp, s0 - 3 + task, L, task);
         else if (task >= A0) angle_bond_task<T, LPB>(D, G, P, mask, Pp, s0 - 2 + (task - A0), L, task - A0);
     }
@@ -153,20 +157,20 @@ __device__ inline void bonded_phase(const Dev &D, const Derived &G, const TermPa
 }
 
 template <typename T, bool EVAL, int LPB>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+__global__ __launch_bounds__(FBLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     const int n = D.nslot;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int t = blk * BLK + threadIdx.x;
+    const int t = blk * FBLK + threadIdx.x;
     const int s = t / LPB, sub = t % LPB;
-    const int s0 = blk * (BLK / LPB);  // the block's first bead
+    const int s0 = blk * (FBLK / LPB);  // the block's first bead
     __shared__ BondedLds<LPB> L;
     // the PID bundle's ss LJ table (depth, r_min, r_max by residue pair;
     // float copies in the FP32 form), read per bundle from LDS
     __shared__ T ss_tab[3 * SS_TYPES];
     {
         const bool f32 = sizeof(T) == 4;
-        for (int q = threadIdx.x; q < SS_TYPES; q += BLK) {
+        for (int q = threadIdx.x; q < SS_TYPES; q += FBLK) {
             ss_tab[q] = f32 ? (T)D.ss_depth_f[q] : (T)D.ss_depth[q];
             ss_tab[SS_TYPES + q] = f32 ? (T)D.ss_rmin_f[q] : (T)D.ss_rmin[q];
             ss_tab[2 * SS_TYPES + q] = f32 ? (T)D.ss_rmax_f[q] : (T)D.ss_rmax[q];

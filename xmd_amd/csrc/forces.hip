@@ -7,13 +7,13 @@ namespace xmd_k {
 template <int LPB>
 static void launch_lpb(bool mixed, bool eval, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
                        const TermTab<float> &Tf, const TermTab<double> &Td, int p, int mutate) {
-    const int grid = nslot * LPB / BLK;
+    const int grid = nslot * LPB / FBLK;
     if (mixed) {
-        if (eval) k_force<float, true, LPB><<<grid, BLK, 0, stream>>>(D, G, Tf, p, mutate);
-        else k_force<float, false, LPB><<<grid, BLK, 0, stream>>>(D, G, Tf, p, mutate);
+        if (eval) k_force<float, true, LPB><<<grid, FBLK, 0, stream>>>(D, G, Tf, p, mutate);
+        else k_force<float, false, LPB><<<grid, FBLK, 0, stream>>>(D, G, Tf, p, mutate);
     } else {
-        if (eval) k_force<double, true, LPB><<<grid, BLK, 0, stream>>>(D, G, Td, p, mutate);
-        else k_force<double, false, LPB><<<grid, BLK, 0, stream>>>(D, G, Td, p, mutate);
+        if (eval) k_force<double, true, LPB><<<grid, FBLK, 0, stream>>>(D, G, Td, p, mutate);
+        else k_force<double, false, LPB><<<grid, FBLK, 0, stream>>>(D, G, Td, p, mutate);
     }
 }
 
