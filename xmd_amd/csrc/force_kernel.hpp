@@ -52,8 +52,20 @@ constexpr int ANG_TERMS[2] = {T_NAT_ANG, T_HEUR_ANG};
 // and one for its angles and bonds, so that no wavefront of the block waits
 // idle for the bonded phase.
 constexpr int FBLK = 128;
+// Partners of a bead are mostly on its own chain, near it in sequence (on
+// evolved 10k-bead boxes of 250-bead chains, 77 % of PID partners lie
+// within 64 beads): each block stages the positions and residues of the
+// beads within WIN_H (plus one) of its own in LDS, and reads a partner and
+// its chain neighbours from there when they fall inside, from global
+// memory otherwise -- the scattered gathers are what occupy the texture
+// path.
+constexpr int WIN_H = 64;
 template <int LPB> struct BondedLds {
     static constexpr int NB = FBLK / LPB, NQ = NB + 3, NA = NB + 2;
+    static constexpr int NW = NB + 2 * WIN_H + 2;  // window beads [s0 - WIN_H - 1, s0 + NB + WIN_H + 1)
+    double2 wxy[NW];
+    double wz[NW];
+    int8_t wat[NW];
     v3<double> qF[NQ][4];
     double qV[NQ][4];
     v3<double> aF[NA][3];
@@ -176,6 +188,16 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
             ss_tab[2 * SS_TYPES + q] = f32 ? (T)D.ss_rmax_f[q] : (T)D.ss_rmax[q];
         }
     }
+    // the window (slots clamped to the slot range: a partner is always in
+    // the bead's own replica, so a clamped slot is never read as one)
+    const int w0 = s0 - WIN_H - 1;
+    for (int q = threadIdx.x; q < BondedLds<LPB>::NW; q += FBLK) {
+        const int slot = min(max(w0 + q, 0), D.nslot - 1);
+        const double *a = D.pos[p] + 4 * (size_t)slot;
+        L.wxy[q] = *reinterpret_cast<const double2 *>(a);
+        L.wz[q] = a[2];
+        L.wat[q] = D.atype[slot];
+    }
     bonded_phase<T, LPB>(D, G, TT.P, TT.mask, D.pos[p], s0, L);  // ends with a barrier
     // a wavefront's beads lie in one 64-aligned slot range, hence one
     // replica: its index (and so the box) is wave-uniform, in scalar registers
@@ -267,7 +289,23 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                     j_n = D.nl_j[e + 64 * LPB];
                 }
                 if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
-                const v3<double> rj = ldp(Pp, j);
+                // the partner and its chain neighbours (read for PID
+                // entries, whose partners have both), and its residue
+                v3<double> rj, rjp, rjn;
+                int aj;
+                const int w = j - w0;
+                if (w >= 1 && w + 1 < BondedLds<LPB>::NW) {
+                    const double2 a = L.wxy[w], b = L.wxy[w - 1], c = L.wxy[w + 1];
+                    rj = {a.x, a.y, L.wz[w]};
+                    rjp = {b.x, b.y, L.wz[w - 1]};
+                    rjn = {c.x, c.y, L.wz[w + 1]};
+                    aj = L.wat[w];
+                } else {
+                    rj = ldp(Pp, j);
+                    rjp = (fl & F_PID) ? ldp(Pp, j - 1) : rj;
+                    rjn = (fl & F_PID) ? ldp(Pp, j + 1) : rj;
+                    aj = D.atype[j];
+                }
                 const v3<double> dd = wrap(box, rj - ri);
                 const v3<T> d = cvt<T>(dd);
                 const bool owner = s < j;
@@ -278,7 +316,7 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                 }
                 if ((fl & F_ES) && (ces_on || res_on)) {
                     // both variants run over the one list when both are on (D31)
-                    T q1q2 = owner ? q_s * (T)D.charge[D.atype[j]] : (T)D.charge[D.atype[j]] * q_s;
+                    T q1q2 = owner ? q_s * (T)D.charge[aj] : (T)D.charge[aj] * q_s;
                     if (ces_on) {
                         FV<T> f = es_pair<T>(d, q1q2, P.es_Vf, P.es_sdi, 0);
                         add3(Fx, Fy, Fz, f.F);
@@ -296,9 +334,8 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                     // two sides, so out[0..2] are the reference's forces on
                     // (s-1, s, s+1) whichever bead of the pair is i1; the
                     // energy is counted by the owner (s = i1).
-                    const v3<double> rjp = ldp(Pp, j - 1), rjn = ldp(Pp, j + 1);
-                    const int i1 = owner ? s : j, i2 = owner ? j : s;
-                    const int ty = 20 * D.atype[i1] + D.atype[i2];
+                    const int a_s = D.atype[s];
+                    const int ty = owner ? 20 * a_s + aj : 20 * aj + a_s;
                     v3<T> out[6];
                     T v = pid_bundle<T>(P, ss_tab[ty], ss_tab[SS_TYPES + ty], ss_tab[2 * SS_TYPES + ty],
                                         pid_side<T>(ri - rsp, rsn - ri, rsn - rsp),
