@@ -97,7 +97,7 @@ def row_entries(eng):
 def force_kernel_bytes(eng, system):
     """Algorithmic bytes of one launch of the per-bead force kernel.
 
-    Per neighbour-row entry: int32 partner + uint8 flags (5 B).  Per native
+    Per neighbour-row entry: one word, partner and flags (4 B).  Per native
     row entry: int32 partner + fp64 distance + uint8 active (13 B).  Per
     bead: its position (32 B, AoS x/y/z/pad; partner gathers re-read
     positions that are already counted), bonded natives (5 fp64 = 40 B),
@@ -108,7 +108,7 @@ def force_kernel_bytes(eng, system):
     e = row_entries(eng)
     nnat = 2 * len(system.nat_i1)
     per_bead = 32 + 40 + 8 + 2 + 8 + 8 + 24 + 48
-    return 5 * e + 13 * nnat + per_bead * n
+    return 4 * e + 13 * nnat + per_bead * n
 
 
 def measured_traffic(kernel, replicas, groups):

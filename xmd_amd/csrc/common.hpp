@@ -17,6 +17,13 @@ constexpr int NUM_TERMS = 18;
 // F_NATSS marks a disulfide pair of the list (native_ssbonds): F_NATIVE is
 // set too, so no non-native term takes it.
 enum : uint8_t { F_NATIVE = 1, F_PAULI = 2, F_ES = 4, F_PID = 8, F_QAFREE = 16, F_NATSS = 32 };
+// A neighbour-row entry packs the partner slot and the flags into one word
+// (one load per entry); slots are below 2^NLE_SHIFT.
+constexpr int NLE_SHIFT = 26;
+constexpr uint32_t NLE_JMASK = (1u << NLE_SHIFT) - 1u;
+__host__ __device__ inline uint32_t nle_pack(int j, uint8_t f) { return (uint32_t)j | ((uint32_t)f << NLE_SHIFT); }
+__host__ __device__ inline int nle_j(uint32_t x) { return (int)(x & NLE_JMASK); }
+__host__ __device__ inline uint8_t nle_flag(uint32_t x) { return (uint8_t)(x >> NLE_SHIFT); }
 // B_HANG / B_HDIH: the model's angle (dihedral) starting at this bead has no
 // native value, so the heuristic term takes it when enabled.
 enum : uint8_t { B_PREV = 1, B_NEXT = 2, B_ACTIVE = 4, B_HANG = 8, B_HDIH = 16 };
@@ -89,8 +96,9 @@ struct Dev {
     const double *vafm_orig, *vafm_v, *fafm_F;
     // neighbour rows
     int nl_w;  // row capacity per slot (rows are fixed-width sliced ELL)
-    int *slice_start, *nl_len, *nl_j;
-    uint8_t *nl_flag;
+    // entry: partner slot in the low NLE_SHIFT bits, term flags above
+    int *slice_start, *nl_len;
+    uint32_t *nl_e;
     // QA free pairs, candidates
     const int *fp_base, *fp_cap, *fp_chunk_rep;
     int *fp_count, *fp_i1, *fp_i2, *fp_slice_cnt;

@@ -484,17 +484,12 @@ struct Part {
     void widen_rows(int W2) {
         const int nsl = nslot / WAVE;
         if ((long long)nslot * W2 >= (1LL << 31)) throw Fail{XMD_ERR_CAPACITY, "rows too wide for int32 offsets"};
-        int *nj = alloc<int>((size_t)nslot * W2);
-        uint8_t *nf = alloc<uint8_t>((size_t)nslot * W2);
-        HIPCHK(hipMemcpy2DAsync(nj, (size_t)WAVE * W2 * sizeof(int), D.nl_j, (size_t)WAVE * nl_w * sizeof(int),
-                                (size_t)WAVE * nl_w * sizeof(int), nsl, hipMemcpyDeviceToDevice, stream));
-        HIPCHK(hipMemcpy2DAsync(nf, (size_t)WAVE * W2, D.nl_flag, (size_t)WAVE * nl_w, (size_t)WAVE * nl_w, nsl,
-                                hipMemcpyDeviceToDevice, stream));
+        uint32_t *ne = alloc<uint32_t>((size_t)nslot * W2);
+        HIPCHK(hipMemcpy2DAsync(ne, (size_t)WAVE * W2 * sizeof(uint32_t), D.nl_e, (size_t)WAVE * nl_w * sizeof(uint32_t),
+                                (size_t)WAVE * nl_w * sizeof(uint32_t), nsl, hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        release(D.nl_j);
-        release(D.nl_flag);
-        D.nl_j = nj;
-        D.nl_flag = nf;
+        release(D.nl_e);
+        D.nl_e = ne;
         D.nl_w = nl_w = W2;
         std::vector<int> ss(nsl);
         for (int sl = 0; sl < nsl; ++sl) ss[sl] = sl * WAVE * W2;
@@ -812,6 +807,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     }
     E.rep_slot_off[E.nrep] = so;
     E.nslot = round_up(std::max(so, BLK), BLK);
+    if (E.nslot > (int)NLE_JMASK)
+        throw Fail{XMD_ERR_CAPACITY, "too many beads for one engine part (a row entry holds a slot in 26 bits)"};
     const int n = E.nslot;
     E.slot_of_bead.resize(E.nbeads);
     E.bead_of_slot.assign(n, -1);
@@ -1229,8 +1226,7 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     E.fp_total = (int)fp;
     E.put(const_cast<int *>(D.fp_base), E.fp_base_h);
     E.put(const_cast<int *>(D.fp_cap), E.fp_cap_h);
-    D.nl_j = E.alloc<int>(nl);
-    D.nl_flag = E.alloc<uint8_t>(nl);
+    D.nl_e = E.alloc<uint32_t>(nl);
     D.fp_i1 = E.alloc<int>(fp);
     D.fp_i2 = E.alloc<int>(fp);
     D.fp_vac = E.alloc<uint8_t>(fp);
@@ -1422,12 +1418,12 @@ static xmd_status part_get_list(Part *e, int32_t rep, int32_t id, int32_t *i1, i
             // the replica's rows: slots [s0, s0 + nb) -> slices of width 64 nl_w
             const size_t r0 = (size_t)(s0 / WAVE) * WAVE * e->nl_w;
             const size_t rn = (size_t)((nb + WAVE - 1) / WAVE) * WAVE * e->nl_w;
-            auto j = e->get(e->D.nl_j + r0, rn);
-            auto fl = e->get(e->D.nl_flag + r0, rn);
+            auto ent = e->get(e->D.nl_e + r0, rn);
             for (int s = s0; s < s0 + nb; ++s)
                 for (int k = 0; k < len[s]; ++k) {
                     size_t idx = (size_t)(s / WAVE) * WAVE * e->nl_w - r0 + 64 * (size_t)k + s % WAVE;
-                    if (j[idx] > s && (fl[idx] & want)) out.push_back({local(s), local(j[idx])});
+                    const int j = nle_j(ent[idx]);
+                    if (j > s && (nle_flag(ent[idx]) & want)) out.push_back({local(s), local(j)});
                 }
         } else if (id == XMD_LIST_QA_FREE) {
             auto cntv = e->get(e->D.fp_count, e->nrep);

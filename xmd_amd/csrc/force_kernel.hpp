@@ -275,19 +275,17 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
             const T q_s = (T)D.charge[D.atype[s]];
             const v3<double> rsp = (bf & B_PREV) ? ldp(Pp, s - 1) : ri;
             const v3<double> rsn = (bf & B_NEXT) ? ldp(Pp, s + 1) : ri;
-            // each lane loads its next entry's flags and partner one
-            // iteration ahead, so an iteration waits on one round of
+            // each lane loads its entries (partner and flags in one word)
+            // two iterations ahead, so an iteration waits on one round of
             // position gathers only
             int e = base + 64 * sub;
-            uint8_t fl_n = sub < len ? D.nl_flag[e] : 0;
-            int j_n = sub < len ? D.nl_j[e] : 0;
+            uint32_t ent_n = sub < len ? D.nl_e[e] : 0u;
+            uint32_t ent_nn = sub + LPB < len ? D.nl_e[e + 64 * LPB] : 0u;
             for (int k = sub; k < len; k += LPB, e += 64 * LPB) {
-                const uint8_t fl = fl_n;
-                const int j = j_n;
-                if (k + LPB < len) {
-                    fl_n = D.nl_flag[e + 64 * LPB];
-                    j_n = D.nl_j[e + 64 * LPB];
-                }
+                const uint8_t fl = nle_flag(ent_n);
+                const int j = nle_j(ent_n);
+                ent_n = ent_nn;
+                if (k + 2 * LPB < len) ent_nn = D.nl_e[e + 128 * LPB];
                 if (!(fl & (F_PAULI | F_ES | F_PID))) continue;
                 // the partner and its chain neighbours (read for PID
                 // entries, whose partners have both), and its residue

@@ -858,8 +858,7 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
             const int j = lj[k];
             int r = 0;
             for (int q = 0; q < cnt; ++q) r += lj[q] < j;
-            D.nl_j[base + 64 * r] = j;
-            D.nl_flag[base + 64 * r] = lf[k];
+            D.nl_e[base + 64 * r] = nle_pack(j, lf[k]);
         }
     }
     if (!active) {
@@ -932,12 +931,13 @@ __global__ __launch_bounds__(BLK) void k_fp_write(Dev D, int p) {
         const int len = (D.bflags[s] & B_ACTIVE) && D.fp_n[s] ? D.nl_len[s] : 0;
         for (int k0 = 0; k0 < len; k0 += WAVE) {
             const int k = k0 + l, e = base + 64 * k;
-            const bool keep = k < len && (D.nl_flag[e] & F_QAFREE);
+            const uint32_t ent = k < len ? D.nl_e[e] : 0u;
+            const bool keep = nle_flag(ent) & F_QAFREE;
             const unsigned long long m = __ballot(keep);
             const int o = off + __popcll(m & ((1ull << l) - 1ull));
             if (keep && o < end) {
                 D.fp_i1[o] = s;
-                D.fp_i2[o] = D.nl_j[e];
+                D.fp_i2[o] = nle_j(ent);
                 D.fp_vac[o] = 0;
             }
             off += __popcll(m);
