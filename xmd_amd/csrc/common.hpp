@@ -117,7 +117,10 @@ struct Dev {
     int *qslot, *qn;
     // tables
     const double *ss_depth, *ss_rmin, *ss_rmax, *req_min_dist, *charge;
-    const float *ss_depth_f, *ss_rmin_f, *ss_rmax_f;
+    // the PID ss LJ table packed per residue pair (depth, r_min, r_max, 0):
+    // one 16- (float, for the FP32 form) or 32-byte (double) load per pair type
+    const float *ss_pack_f;
+    const double *ss_pack;
     const int8_t *ptype;
     const uint8_t *bm_fix;  // glibc correction table and its thresholds (bead_noise)
     const float *bm_thr;

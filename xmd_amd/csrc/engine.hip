@@ -997,11 +997,13 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     std::vector<double> ss_depth(P.ss_depth, P.ss_depth + XMD_NUM_SS), ss_rmax(P.ss_rmax, P.ss_rmax + XMD_NUM_SS),
         ss_rmin(XMD_NUM_SS);
     for (int k = 0; k < XMD_NUM_SS; ++k) ss_rmin[k] = P.ss_sinking ? P.lj_bb_rmin : ss_rmax[k];
-    std::vector<float> ssdf(XMD_NUM_SS), ssnf(XMD_NUM_SS), ssxf(XMD_NUM_SS);
+    std::vector<float> sspf(4 * XMD_NUM_SS, 0.0f);
+    std::vector<double> ssp(4 * XMD_NUM_SS, 0.0);
     for (int k = 0; k < XMD_NUM_SS; ++k) {
-        ssdf[k] = (float)ss_depth[k];
-        ssnf[k] = (float)ss_rmin[k];
-        ssxf[k] = (float)ss_rmax[k];
+        ssp[4 * k] = ss_depth[k];
+        ssp[4 * k + 1] = ss_rmin[k];
+        ssp[4 * k + 2] = ss_rmax[k];
+        for (int c = 0; c < 3; ++c) sspf[4 * k + c] = (float)ssp[4 * k + c];
     }
     std::vector<double> rmd(XMD_QA_NUM_TYPES, 0.0);
     rmd[1] = P.lj_bb_rmin;
@@ -1117,9 +1119,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.ss_depth = E.upload(ss_depth);
     D.ss_rmin = E.upload(ss_rmin);
     D.ss_rmax = E.upload(ss_rmax);
-    D.ss_depth_f = E.upload(ssdf);
-    D.ss_rmin_f = E.upload(ssnf);
-    D.ss_rmax_f = E.upload(ssxf);
+    D.ss_pack_f = E.upload(sspf);
+    D.ss_pack = E.upload(ssp);
     D.req_min_dist = E.upload(rmd);
     D.charge = E.upload(charge);
     D.ptype = E.upload(ptype);

@@ -179,14 +179,11 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
     __shared__ BondedLds<LPB> L;
     // the PID bundle's ss LJ table (depth, r_min, r_max by residue pair;
     // float copies in the FP32 form), read per bundle from LDS
-    __shared__ T ss_tab[3 * SS_TYPES];
+    using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+    __shared__ T4 ss_tab[SS_TYPES];
     {
-        const bool f32 = sizeof(T) == 4;
-        for (int q = threadIdx.x; q < SS_TYPES; q += FBLK) {
-            ss_tab[q] = f32 ? (T)D.ss_depth_f[q] : (T)D.ss_depth[q];
-            ss_tab[SS_TYPES + q] = f32 ? (T)D.ss_rmin_f[q] : (T)D.ss_rmin[q];
-            ss_tab[2 * SS_TYPES + q] = f32 ? (T)D.ss_rmax_f[q] : (T)D.ss_rmax[q];
-        }
+        const T4 *src = reinterpret_cast<const T4 *>(sizeof(T) == 4 ? (const void *)D.ss_pack_f : (const void *)D.ss_pack);
+        for (int q = threadIdx.x; q < SS_TYPES; q += FBLK) ss_tab[q] = src[q];
     }
     // the window (slots clamped to the slot range: a partner is always in
     // the bead's own replica, so a clamped slot is never read as one)
@@ -335,7 +332,8 @@ void k_force(Dev D, Derived G, TermTab<T> TT, int p, int mutate) {
                     const int a_s = D.atype[s];
                     const int ty = owner ? 20 * a_s + aj : 20 * aj + a_s;
                     v3<T> out[6];
-                    T v = pid_bundle<T>(P, ss_tab[ty], ss_tab[SS_TYPES + ty], ss_tab[2 * SS_TYPES + ty],
+                    const T4 sst = ss_tab[ty];
+                    T v = pid_bundle<T>(P, sst.x, sst.y, sst.z,
                                         pid_side<T>(ri - rsp, rsn - ri, rsn - rsp),
                                         pid_side<T>(rj - rjp, rjn - rj, rjn - rjp), dd, out);
                     add3(Fx, Fy, Fz, out[1]);
