@@ -703,12 +703,18 @@ __global__ __launch_bounds__(BLK) void k_cell_fill(Dev D, int p) {
 // Row flags of the pair (s, j) at rebuild: legacy_update's criterion
 // (legacy_update.cpp:24-29) and the per-term filters of update_X
 // (pauli.cpp:13-20, es_pair.cpp:9-21, bundle.cpp:9-26, free_pair.cpp:9-19).
-__device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &box, int s, int j, v3<double> rs,
-                                     const double *P) {
-    if (D.chain[j] == D.chain[s] && (s > j ? s - j : j - s) < 3) return 0;
+// The pair is a list candidate: not a chain neighbour within 2, closer than
+// the list radius.
+__device__ inline bool pair_near(const Dev &D, const Derived &G, const Box &box, int s, int j, v3<double> rs,
+                                 const double *P) {
+    if (D.chain[j] == D.chain[s] && (s > j ? s - j : j - s) < 3) return false;
+    return norm(wrap(box, ldp(P, j) - rs)) < G.req_r;
+}
+// Flags of a pair that pair_near accepts.
+__device__ inline uint8_t pair_flags_near(const Dev &D, const Derived &G, const Box &box, int s, int j, v3<double> rs,
+                                          const double *P) {
     v3<double> d = wrap(box, ldp(P, j) - rs);
     double dist = norm(d);
-    if (!(dist < G.req_r)) return 0;
     // disulfide pair: in native_ssbonds only, no pair term (D24)
     for (int k = D.ss_start[s]; k < D.ss_start[s + 1]; ++k)
         if (D.ss_j[k] == j) return F_NATIVE | F_NATSS;
@@ -739,6 +745,10 @@ __device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &
         if (!in_contact) f |= F_QAFREE;
     }
     return f;
+}
+__device__ inline uint8_t pair_flags(const Dev &D, const Derived &G, const Box &box, int s, int j, v3<double> rs,
+                                     const double *P) {
+    return pair_near(D, G, box, s, j, rs, P) ? pair_flags_near(D, G, box, s, j, rs, P) : 0;
 }
 
 // Wavefront per slot: the lanes share the slot's candidate partners.  With
@@ -832,18 +842,38 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
     const int W = D.nl_w;
     int cnt = 0, nfp = 0;
     if (active) {
+        // the candidates within the list radius first (most of the 27
+        // cells' beads are not), compacted in candidate order into lj; then
+        // their flags, the kept ones compacted in place (a chunk's entries
+        // are read before any of its writes, which land below the next chunk)
+        int nnear = 0;
         for_candidates_wave(D, s, rep, [&](int j) {
-            uint8_t f = j >= 0 ? pair_flags(D, G, box, s, j, rs, P) : 0;
-            bool keep = f & (F_PAULI | F_ES | F_PID | F_QAFREE);
-            unsigned long long m = __ballot(keep);
-            int pos = cnt + __popcll(m & ((1ull << l) - 1ull));
+            const bool near = j >= 0 && pair_near(D, G, box, s, j, rs, P);
+            const unsigned long long m = __ballot(near);
+            const int pos = nnear + __popcll(m & ((1ull << l) - 1ull));
+            if (near && pos < MAXSORT) lj[pos] = j;
+            nnear += __popcll(m);
+        });
+        if (nnear > MAXSORT) {
+            if (l == 0) {
+                atomicOr(D.err + rep, E_NL_CAP);
+                atomicMax(D.need_w + rep, nnear);
+            }
+            nnear = MAXSORT;
+        }
+        for (int k0 = 0; k0 < nnear; k0 += WAVE) {
+            const int j = k0 + l < nnear ? lj[k0 + l] : -1;
+            const uint8_t f = j >= 0 ? pair_flags_near(D, G, box, s, j, rs, P) : 0;
+            const bool keep = f & (F_PAULI | F_ES | F_PID | F_QAFREE);
+            const unsigned long long m = __ballot(keep);
+            const int pos = cnt + __popcll(m & ((1ull << l) - 1ull));
             if (keep && pos < W) {
                 lj[pos] = j;
                 lf[pos] = f;
                 nfp += (f & F_QAFREE) != 0;
             }
             cnt += __popcll(m);
-        });
+        }
         if (cnt > W) {
             if (l == 0) {
                 atomicOr(D.err + rep, E_NL_CAP);
