@@ -63,7 +63,9 @@ struct Dev {
     const double *noise_sd;
     uint64_t *rng[2];
     double *tt[2];
-    unsigned long long *maxd[2];
+    // largest displacement since the list build of each 64-slot slice (bit
+    // pattern of a non-negative double, NaN above all), per parity
+    unsigned long long *slice_maxd[2];
     int *rb_now, *force_rb, *any_rb, *err;
     long long *n_rebuild;
     long long *n_done;  // steps each replica has taken (halted ones stop counting)

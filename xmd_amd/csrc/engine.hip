@@ -1058,8 +1058,8 @@ void build(Part &E, const xmd_params *p, const xmd_topology *t, double go_cut) {
     D.rng[1] = E.upload(rng);
     D.tt[0] = E.upload(t0);
     D.tt[1] = E.upload(t0);
-    D.maxd[0] = E.alloc<unsigned long long>(E.nrep);
-    D.maxd[1] = E.alloc<unsigned long long>(E.nrep);
+    D.slice_maxd[0] = E.alloc<unsigned long long>(n / WAVE);
+    D.slice_maxd[1] = E.alloc<unsigned long long>(n / WAVE);
     D.rb_now = E.alloc<int>(E.nrep);
     D.force_rb = E.alloc<int>(E.nrep);
     D.any_rb = E.alloc<int>(2);

@@ -442,13 +442,16 @@ __device__ inline void integrate_bead_impl(const Dev &D, const Derived &G, int p
         disp = norm(dr);
         if (!(disp == disp) || !(fabs(y0.x) < 1e300)) disp = __builtin_nan("");
     }
-    // wave max (non-negative doubles order like their bit patterns; NaN wins)
+    // wave max (non-negative doubles order like their bit patterns; NaN
+    // wins), stored for the wavefront's slice: a plain store, where an
+    // atomic per wavefront on one word per replica would serialise at the
+    // memory side (the XCDs' L2s are not coherent)
     unsigned long long bits = active ? (unsigned long long)xmd_bm::dbits(disp) : 0ull;
     for (int o = 32; o > 0; o >>= 1) {
         unsigned long long other = __shfl_xor(bits, o);
         bits = other > bits ? other : bits;
     }
-    if (lane_id() == 0 && bits) atomicMax(D.maxd[p] + rep, bits);
+    if (lane_id() == 0) D.slice_maxd[p][s >> 6] = bits;
 }
 
 // ------------------------------------------------------------- forces ---
@@ -519,7 +522,6 @@ __device__ inline int finalize_wave(const Dev &D, const Derived &G, int p, int m
         if (l == 0) {
             D.rng[1 - p][rep] = D.rng[p][rep];
             D.tt[1 - p][rep] = D.tt[p][rep];
-            D.maxd[1 - p][rep] = 0ull;
             D.rb_now[rep] = 0;
         }
         return 0;
@@ -532,18 +534,29 @@ __device__ inline int finalize_wave(const Dev &D, const Derived &G, int p, int m
             D.n_done[rep] += 1;
         }
     }
+    // the replica's largest displacement: the maximum over its slices,
+    // each written by the integrator of this step (a running replica's
+    // every wavefront stores its slice)
+    unsigned long long mb = 0ull;
+    if (mode == 1) {
+        const int sl0 = D.rep_slot_off[rep] >> 6, sl1 = (D.rep_slot_off[rep] + D.rep_n[rep] + WAVE - 1) >> 6;
+        for (int t = sl0 + l; t < sl1; t += WAVE) {
+            const unsigned long long v = D.slice_maxd[p][t];
+            mb = v > mb ? v : mb;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long other = __shfl_xor(mb, o);
+            mb = other > mb ? other : mb;
+        }
+    }
     int rb = 0;
     if (l == 0) {
         rb = D.force_rb[rep];
         if (mode == 1) {
-            double m = xmd_bm::bitsd(D.maxd[p][rep]);
+            double m = xmd_bm::bitsd(mb);
             if (!(m == m)) atomicOr(D.err + rep, E_NONFINITE);
             double total = 2.0 * m + 0.0;
             if (total > G.pad) rb = 1;
-            D.maxd[1 - p][rep] = 0ull;
-        } else {
-            D.maxd[p][rep] = 0ull;
-            D.maxd[1 - p][rep] = 0ull;
         }
         D.rb_now[rep] = rb;
         D.force_rb[rep] = 0;
