@@ -185,8 +185,9 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
     }
 }
 
-__device__ inline void slot_add(const Dev &D, int s, int ci) {
-    int q = D.qn[s];
+// Contact slot lists of a bead: slot_add_q appends ci at position q (the
+// bead's current count), slot_remove takes ci out keeping the order.
+__device__ inline void slot_add_q(const Dev &D, int s, int q, int ci) {
     if (q >= MAXQ) {
         atomicOr(D.err + D.slot_rep[s], E_SLOT);
         return;
@@ -201,41 +202,107 @@ __device__ inline void slot_remove(const Dev &D, int s, int ci) {
     for (; k + 1 < q; ++k) D.qslot[k * D.nslot + s] = D.qslot[(k + 1) * D.nslot + s];
     D.qn[s] = q - 1;
 }
-
-// Collect, in order, the chunks [c_lo, c_hi) whose counter is non-zero into
-// s_list (at most BLK per call, starting at chunk `from`); returns how many
-// were listed and sets *next to the first chunk not examined.
-__device__ inline int list_nonzero_chunks(const int *cnt, int c_lo, int c_hi, int from, int *s_list, int *s_n,
-                                          int *next) {
-    __shared__ int s_w[BLK / WAVE];
-    int tid = threadIdx.x;
-    int c = from + tid;
-    bool nz = c < c_hi && cnt[c] > 0;
-    unsigned long long m = __ballot(nz);
-    int w = tid / WAVE;
-    if (lane_id() == 0) s_w[w] = __popcll(m);
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int k = 0; k < BLK / WAVE; ++k) {
-        if (k < w) off += s_w[k];
-        tot += s_w[k];
+// One contact removal (process_contacts.cpp): the pair's budgets return to
+// its beads, the pair rejoins the free pairs at position o of the
+// replica's set (or flags the overflow), and leaves its beads' slot lists.
+__device__ inline void remove_contact(const Dev &D, int rep, int ci, int i1, int i2, int fpb, int o, int fcap) {
+    D.ct_vac[ci] = 1;
+    D.sync[i1] = sync_add(D.sync[i1], D.ct_d1[ci]);
+    D.sync[i2] = sync_add(D.sync[i2], D.ct_d2[ci]);
+    if (o >= fcap) {
+        atomicOr(D.err + rep, E_FP_CAP);
+    } else {
+        const int f = fpb + o;
+        D.fp_i1[f] = i1;
+        D.fp_i2[f] = i2;
+        D.fp_vac[f] = 0;
     }
-    if (nz) s_list[off + __popcll(m & ((1ull << lane_id()) - 1ull))] = c;
-    if (tid == 0) *s_n = tot;
-    __syncthreads();
-    *next = from + BLK;
-    (void)c_lo;
-    return *s_n;
+    slot_remove(D, i1, ci);
+    slot_remove(D, i2, ci);
 }
+
+// List, in order, the chunks [from, c_hi) whose counter is non-zero:
+// s_list[q] is the chunk, s_off[q] the sum of the counters of the chunks
+// listed before it (s_off[n] the total).  LIST_PER consecutive chunks per
+// thread, their counters loaded together.  Returns the number listed n
+// and sets *next to the first chunk not examined.
+constexpr int LIST_PER = 8, LIST_SPAN = LIST_PER * BLK;
+__device__ inline int list_nonzero_chunks(const int *cnt, int c_hi, int from, int *s_list, int *s_off, int *next) {
+    __shared__ int s_w[2][BLK / WAVE];
+    const int tid = threadIdx.x, c0 = from + LIST_PER * tid;
+    int v[LIST_PER];
+#pragma unroll
+    for (int i = 0; i < LIST_PER; ++i) v[i] = c0 + i < c_hi ? cnt[c0 + i] : 0;
+    int mine = 0, ments = 0;
+#pragma unroll
+    for (int i = 0; i < LIST_PER; ++i) {
+        mine += v[i] > 0;
+        ments += v[i];
+    }
+    // exclusive scans of the per-thread chunk and entry counts: wave
+    // shuffles, then waves
+    int x = mine, y = ments;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int xo = __shfl_up(x, o), yo = __shfl_up(y, o);
+        if (lane_id() >= o) {
+            x += xo;
+            y += yo;
+        }
+    }
+    const int w = tid / WAVE;
+    if (lane_id() == WAVE - 1) {
+        s_w[0][w] = x;
+        s_w[1][w] = y;
+    }
+    __syncthreads();
+    int off = x - mine, eoff = y - ments, tot = 0, etot = 0;
+    for (int k = 0; k < BLK / WAVE; ++k) {
+        if (k < w) {
+            off += s_w[0][k];
+            eoff += s_w[1][k];
+        }
+        tot += s_w[0][k];
+        etot += s_w[1][k];
+    }
+#pragma unroll
+    for (int i = 0; i < LIST_PER; ++i)
+        if (v[i] > 0) {
+            s_list[off] = c0 + i;
+            s_off[off++] = eoff;
+            eoff += v[i];
+        }
+    if (tid == 0) s_off[tot] = etot;
+    __syncthreads();
+    *next = from + LIST_SPAN;
+    return tot;
+}
+// The listed chunk holding entry k (0 <= k < s_off[n]): the largest q with
+// s_off[q] <= k (listed chunks are non-empty, so the offsets increase).
+__device__ inline int listed_chunk(const int *s_off, int n, int k) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_off[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One batch of candidates or removals staged in LDS.
+struct QaBatch {
+    int e[BLK], i1[BLK], i2[BLK], ty[BLK], q1[BLK], q2[BLK], dup[BLK];
+    uint32_t d1[BLK], d2[BLK], y1[BLK], y2[BLK];
+};
 
 // One workgroup per replica: process candidates in free-pair order, then
 // the removals of process_contacts in contact order, then compact the
 // contact set when its extent passes half the capacity.  The workgroup
-// lists the non-empty chunks; one thread applies their entries serially.
+// lists the non-empty chunks and stages their entries; the order-dependent
+// steps run on one thread.
 __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
     __shared__ int s_scan[BLK];
-    __shared__ int s_list[BLK];
-    __shared__ int s_n;
+    __shared__ int s_list[LIST_SPAN], s_off[LIST_SPAN + 1];
+    __shared__ QaBatch s_b;
     __shared__ int s_state[4];  // ct count, fp count, cnt0
     int rep = blockIdx.x, tid = threadIdx.x;
     if (D.err[rep]) return;  // halted replica: its QA state stays as it is
@@ -249,30 +316,64 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
     }
     __syncthreads();
     const int cap = D.ct_cap[rep], tot = D.ct_cap[D.nrep], fcap = D.fp_cap[rep];
-    // candidates, in free-pair order
+    // candidates, in free-pair order.  The entries of the listed chunks
+    // are staged in batches of BLK (pair, budgets, slot counts, loaded by
+    // the whole block), then one thread accepts them in order from LDS; a
+    // bead that an earlier candidate of the batch names is read back from
+    // memory, where that thread wrote it.  Once the contact set is full
+    // every later acceptance fails alike (the error is set, nothing else
+    // changes).
     for (int from = fc_lo; from < fc_hi;) {
         int nxt;
-        int nl = list_nonzero_chunks(D.cand_cnt, fc_lo, fc_hi, from, s_list, &s_n, &nxt);
-        if (tid == 0) {
-            int cnt = s_state[0];
-            for (int q = 0; q < nl; ++q) {
-                int chunk = s_list[q];
-                int nc = D.cand_cnt[chunk];
-                for (int k = 0; k < nc; ++k) {
-                    int e = chunk * BLK + k;
-                    int fp = D.cand_fp[e];
-                    int i1 = D.fp_i1[fp], i2 = D.fp_i2[fp];
-                    uint32_t d1 = D.cand_d1[e], d2 = D.cand_d2[e];
-                    uint32_t ns1 = sync_sub(D.sync[i1], d1), ns2 = sync_sub(D.sync[i2], d2);
+        const int nl = list_nonzero_chunks(D.cand_cnt, fc_hi, from, s_list, s_off, &nxt);
+        const int ne = s_off[nl];
+        for (int b0 = 0; b0 < ne; b0 += BLK) {
+            const int k = b0 + tid, nb = min(BLK, ne - b0);
+            int i1 = -1, i2 = -1;
+            if (k < ne) {
+                const int q = listed_chunk(s_off, nl, k);
+                const int e = s_list[q] * BLK + (k - s_off[q]);
+                const int fp = D.cand_fp[e];
+                i1 = D.fp_i1[fp];
+                i2 = D.fp_i2[fp];
+                s_b.e[tid] = fp;
+                s_b.i1[tid] = i1;
+                s_b.i2[tid] = i2;
+                s_b.ty[tid] = D.cand_type[e];
+                s_b.d1[tid] = D.cand_d1[e];
+                s_b.d2[tid] = D.cand_d2[e];
+                s_b.y1[tid] = D.sync[i1];
+                s_b.y2[tid] = D.sync[i2];
+                s_b.q1[tid] = D.qn[i1];
+                s_b.q2[tid] = D.qn[i2];
+            }
+            __syncthreads();
+            if (k < ne) {
+                int dup = 0;
+                for (int m = 0; m < tid; ++m) {
+                    const int a = s_b.i1[m], b = s_b.i2[m];
+                    dup |= (a == i1 || b == i1) ? 1 : 0;
+                    dup |= (a == i2 || b == i2) ? 2 : 0;
+                }
+                s_b.dup[tid] = dup;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int cnt = s_state[0];
+                for (int m = 0; m < nb; ++m) {
+                    const int a1 = s_b.i1[m], a2 = s_b.i2[m], dup = s_b.dup[m];
+                    const uint32_t d1 = s_b.d1[m], d2 = s_b.d2[m];
+                    const uint32_t ns1 = sync_sub(dup & 1 ? D.sync[a1] : s_b.y1[m], d1);
+                    const uint32_t ns2 = sync_sub(dup & 2 ? D.sync[a2] : s_b.y2[m], d2);
                     if (!(sync_valid(ns1) && sync_valid(ns2))) continue;
                     if (cnt >= cap) {
                         atomicOr(D.err + rep, E_CT_CAP);
-                        break;
+                        continue;
                     }
-                    int ci = ctb + cnt++;
-                    D.ct_i1[ci] = i1;
-                    D.ct_i2[ci] = i2;
-                    D.ct_type[ci] = D.cand_type[e];
+                    const int ci = ctb + cnt++;
+                    D.ct_i1[ci] = a1;
+                    D.ct_i2[ci] = a2;
+                    D.ct_type[ci] = s_b.ty[m];
                     D.ct_status[ci] = 0;
                     D.ct_ref[ci] = t;
                     D.ct_d1[ci] = d1;
@@ -282,49 +383,60 @@ __global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
                     D.ct_F[tot + ci] = 0;
                     D.ct_F[2 * tot + ci] = 0;
                     D.ct_V[ci] = 0;
-                    D.sync[i1] = ns1;
-                    D.sync[i2] = ns2;
-                    slot_add(D, i1, ci);
-                    slot_add(D, i2, ci);
-                    D.fp_vac[fp] = 1;
+                    D.sync[a1] = ns1;
+                    D.sync[a2] = ns2;
+                    slot_add_q(D, a1, dup & 1 ? D.qn[a1] : s_b.q1[m], ci);
+                    slot_add_q(D, a2, dup & 2 ? D.qn[a2] : s_b.q2[m], ci);
+                    D.fp_vac[s_b.e[m]] = 1;
                 }
+                s_state[0] = cnt;
             }
-            s_state[0] = cnt;
+            __syncthreads();
         }
-        __syncthreads();
         from = nxt;
     }
-    // removals, in contact order (contacts formed above have none)
+    // removals, in contact order (contacts formed above have none), in
+    // batches of BLK: a removal whose beads no other removal of the batch
+    // names runs on its own thread; the others run in order on one thread.
+    // Removal o of the pass (counted from its start) puts its pair back at
+    // free-pair position fp_count + o, or flags the overflow.
     int cc_lo = ctb / BLK, cc_hi = cc_lo + (s_state[2] + BLK - 1) / BLK;
     for (int from = cc_lo; from < cc_hi;) {
         int nxt;
-        int nl = list_nonzero_chunks(D.rm_cnt, cc_lo, cc_hi, from, s_list, &s_n, &nxt);
-        if (tid == 0) {
-            int fcnt = s_state[1];
-            for (int q = 0; q < nl; ++q) {
-                int chunk = s_list[q];
-                int nr = D.rm_cnt[chunk];
-                for (int k = 0; k < nr; ++k) {
-                    int ci = D.rm_idx[chunk * BLK + k];
-                    int i1 = D.ct_i1[ci], i2 = D.ct_i2[ci];
-                    D.ct_vac[ci] = 1;
-                    D.sync[i1] = sync_add(D.sync[i1], D.ct_d1[ci]);
-                    D.sync[i2] = sync_add(D.sync[i2], D.ct_d2[ci]);
-                    if (fcnt >= fcap) {
-                        atomicOr(D.err + rep, E_FP_CAP);
-                    } else {
-                        int f = fpb + fcnt++;
-                        D.fp_i1[f] = i1;
-                        D.fp_i2[f] = i2;
-                        D.fp_vac[f] = 0;
-                    }
-                    slot_remove(D, i1, ci);
-                    slot_remove(D, i2, ci);
-                }
-                D.rm_cnt[chunk] = 0;
+        const int nl = list_nonzero_chunks(D.rm_cnt, cc_hi, from, s_list, s_off, &nxt);
+        const int ne = s_off[nl];
+        for (int b0 = 0; b0 < ne; b0 += BLK) {
+            const int k = b0 + tid, nb = min(BLK, ne - b0);
+            const int fc = s_state[1];
+            int ci = -1, i1 = -1, i2 = -1;
+            if (k < ne) {
+                const int q = listed_chunk(s_off, nl, k);
+                ci = D.rm_idx[s_list[q] * BLK + (k - s_off[q])];
+                i1 = D.ct_i1[ci];
+                i2 = D.ct_i2[ci];
+                s_b.e[tid] = ci;
+                s_b.i1[tid] = i1;
+                s_b.i2[tid] = i2;
             }
-            s_state[1] = fcnt;
+            __syncthreads();
+            if (k < ne) {
+                bool shared = false;
+                for (int m = 0; m < nb; ++m) {
+                    const int a = s_b.i1[m], b = s_b.i2[m];
+                    shared |= m != tid && (a == i1 || b == i1 || a == i2 || b == i2);
+                }
+                s_b.dup[tid] = shared;
+                if (!shared) remove_contact(D, rep, ci, i1, i2, fpb, fc + tid, fcap);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                for (int m = 0; m < nb; ++m)
+                    if (s_b.dup[m]) remove_contact(D, rep, s_b.e[m], s_b.i1[m], s_b.i2[m], fpb, fc + m, fcap);
+                s_state[1] = min(fcap, fc + nb);
+            }
+            __syncthreads();
         }
+        for (int q = tid; q < nl; q += BLK) D.rm_cnt[s_list[q]] = 0;
         __syncthreads();
         from = nxt;
     }
@@ -844,9 +956,9 @@ __global__ __launch_bounds__(BLK) void k_nl_count(Dev D, Derived G, const double
 // Gather each row in LDS (at most the row width; longer rows flag a
 // capacity error), rank-sort it by partner (partners are distinct), write
 // it; count the half row and the QA free pairs; snapshot orig_r and the Go
-// activity.
-__device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double *P, int s, int rep, int *lj,
-                                    uint8_t *lf) {
+// activity.  Returns the slot's QA free pairs to every lane.
+__device__ inline int nl_fill_slot(const Dev &D, const Derived &G, const double *P, int s, int rep, int *lj,
+                                   uint8_t *lf) {
     const int l = lane_id();
     const bool active = D.bflags[s] & B_ACTIVE;
     const Box box = D.box[rep];
@@ -897,10 +1009,18 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
     }
     __syncthreads();  // reached by every wave of the block: the skip test is block-uniform
     if (active) {
+        // the rank of each entry: every lane reads the same four entries per
+        // LDS load (a broadcast)
+        const int c4 = cnt & ~3;
+        const int4 *lj4 = reinterpret_cast<const int4 *>(lj);
         for (int k = l; k < cnt; k += WAVE) {
             const int j = lj[k];
             int r = 0;
-            for (int q = 0; q < cnt; ++q) r += lj[q] < j;
+            for (int q = 0; q < c4; q += 4) {
+                const int4 v = lj4[q >> 2];
+                r += (v.x < j) + (v.y < j) + (v.z < j) + (v.w < j);
+            }
+            for (int q = c4; q < cnt; ++q) r += lj[q] < j;
             D.nl_e[base + 64 * r] = nle_pack(j, lf[k]);
         }
     }
@@ -909,7 +1029,7 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
             D.nl_len[s] = 0;
             D.fp_n[s] = 0;
         }
-        return;
+        return 0;
     }
     for (int o = 32; o > 0; o >>= 1) {
         nfp += __shfl_xor(nfp, o);
@@ -917,7 +1037,6 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
     if (l == 0) {
         D.nl_len[s] = cnt;
         D.fp_n[s] = nfp;
-        if (nfp) atomicAdd(D.fp_slice_cnt + (s >> 6), nfp);
         stp(D.orig_r, s, rs);
     }
     // Go activity (go.cpp:11-22)
@@ -932,18 +1051,30 @@ __device__ inline void nl_fill_slot(const Dev &D, const Derived &G, const double
         v3<double> d = wrap(box, ldp(P, D.ss_j[k]) - rs);
         D.ss_act[k] = norm(d) < G.ss_cut + G.pad;
     }
+    return nfp;
 }
 
 __global__ __launch_bounds__(BLK) void k_nl_fill(Dev D, Derived G, const double *P, int p) {
-    __shared__ int s_j[NL_WAVES][MAXSORT];
+    __shared__ __align__(16) int s_j[NL_WAVES][MAXSORT];
     __shared__ uint8_t s_f[NL_WAVES][MAXSORT];
+    __shared__ int s_nfp[NL_WAVES];
     if (!D.any_rb[p]) return;
     const int w = threadIdx.x / WAVE;
     for (int s0 = blockIdx.x * NL_WAVES; s0 < D.nslot; s0 += gridDim.x * NL_WAVES) {
         const int s = s0 + w;
         const int rep = D.slot_rep[s];
         if (!D.rb_now[rep]) continue;  // uniform over the block
-        nl_fill_slot(D, G, P, s, rep, s_j[w], s_f[w]);
+        const int nfp = nl_fill_slot(D, G, P, s, rep, s_j[w], s_f[w]);
+        // the block's slots share a 64-slot slice: one add to its free-pair
+        // count per block (the next iteration's writes of s_nfp follow the
+        // barrier inside nl_fill_slot, which thread 0 reaches after its read)
+        if (lane_id() == 0) s_nfp[w] = nfp;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < NL_WAVES; ++k) t += s_nfp[k];
+            if (t) atomicAdd(D.fp_slice_cnt + (s0 >> 6), t);
+        }
     }
 }
 
