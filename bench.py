@@ -256,6 +256,9 @@ def main():
     ap.add_argument("--force-lanes", type=int, default=0, choices=(0, 4, 8, 16),
                     help="lanes per bead of the force kernel instead of the autotuner's choice (0: autotune); "
                          "for counter runs, whose serialized kernels change the autotuner's timings")
+    ap.add_argument("--seed-horizon", type=int, default=0, metavar="STEPS",
+                    help="pick the batch's seeds free of zero log() draws over STEPS steps (0: the steps of this "
+                         "run); the profile commands pass the default run's 4464 so that they step its replicas")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy and a scale kernel of MIB MiB each "
                          "(known HBM bytes, to calibrate FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -271,7 +274,9 @@ def main():
     t0 = time.time()
     # seeds whose noise would hit a zero log() draw inside the run are
     # skipped: the reference's noise is infinite there (rand_gen.h:45-46)
-    horizon = args.warmup + args.steps + args.profile_steps + 64
+    # every engine step of the default run (warm-up, profiled, both windows);
+    # a shorter run given the default run's horizon steps the same seeds
+    horizon = args.seed_horizon or args.warmup + args.profile_steps + args.steps + args.long_steps + 64
     batch = synth.config_system(2, n_batch=args.replicas, horizon_steps=horizon)
     single = synth.config_system(2, n_batch=1, horizon_steps=horizon + max(200, args.steps // 4))
     log(f"systems generated in {time.time() - t0:.1f}s: batch {batch.n_beads} beads, single {single.n_beads}")
