@@ -399,15 +399,19 @@ xmd_status xmd_observables(xmd_engine *eng, double active_threshold, double *out
 #define XMD_OBS_NOALLOC_MAX_REPLICAS 4096
 xmd_status xmd_observables_noalloc(xmd_engine *eng, double active_threshold, double *out);
 
-/* Autotuner (no counterpart in the reference): times `reps` launches of the
- * force kernel with 4, 8 and 16 lanes per bead on the current state, every
- * stream group's launches side by side as in a step, in three interleaved
- * rounds; keeps the candidate with the lowest median for every later step
- * (8, the default, unless the lowest is more than 3 % below its median: the
- * tiles sum a bead's partial forces in different orders, so trajectories
- * differ in rounding between tiles) and returns it in *lpb_out and the three
- * medians (ms per step: the span of one launch of every group) in ms_out[3]
- * (either may be NULL).  The simulation state is not changed. */
+/* Autotuner (no counterpart in the reference): times 16 * reps graph-captured
+ * steps (after 16 untimed ones) with 4, 8 and 16 lanes per bead of the force
+ * kernel, every stream group stepped side by side as xmd_step does, each
+ * sample from the current state (the engine's device state is copied first,
+ * restored before every sample and at the end: it needs as much free device
+ * memory as the engine holds), in three interleaved rounds; keeps the
+ * candidate with the lowest median for every later step (8, the default,
+ * unless the lowest is more than 3 % below its median: the tiles sum a
+ * bead's partial forces in different orders, so trajectories differ in
+ * rounding between tiles) and returns it in *lpb_out and the three medians
+ * (ms per step) in ms_out[3] (either may be NULL).  A sample in which a
+ * replica halts counts as infinitely slow.  The simulation state is not
+ * changed. */
 xmd_status xmd_autotune(xmd_engine *eng, int32_t reps, int32_t *lpb_out, double *ms_out);
 #ifdef __cplusplus
 }

@@ -250,8 +250,10 @@ def test_force_lanes_variants(c2_small, lanes):
 
 
 def test_autotune_keeps_state():
-    """The autotuner times force launches only: the state is unchanged and
-    the trajectory stays in parity with the oracle afterwards."""
+    """The autotuner times whole steps from a copy of the engine's state and
+    restores it: the state is unchanged (positions, Gear derivatives,
+    generator, time) and the trajectory stays in parity with the oracle
+    afterwards."""
     system = synth.config_system(2, n_batch=2, chains=6, horizon_steps=60)
     params = synth.config_params(2)
     eng = Engine(params, system)

@@ -5,10 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -180,6 +182,7 @@ struct Part {
     Dev D;
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
+    std::vector<size_t> alloc_bytes;  // size of each allocation, in allocs' order
     int cur = 0;
     std::string err;
     int rep0 = 0;               // global index of the part's first replica
@@ -196,17 +199,6 @@ struct Part {
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr}, g_one[2] = {nullptr, nullptr};
     int chunk_steps = 16;
     int lpb = FORCE_LPB;  // lanes per bead of the force kernel
-    // Queues `reps` force-kernel launches with `l` lanes per bead on the
-    // current state between events a and b (recorded on the part's stream).
-    // The kernel only writes the force buffers, which every step
-    // recomputes, so the simulation state is untouched.
-    void queue_force(int l, int reps, hipEvent_t a, hipEvent_t b) {
-        const bool mixed = P.precision == XMD_PREC_MIXED;
-        HIPCHK(hipEventRecord(a, stream));
-        for (int k = 0; k < reps; ++k) launch_force(mixed, false, l, nslot, stream, D, G, TTf, TTd, cur, 0);
-        HIPCHK(hipEventRecord(b, stream));
-        HIPCHK(hipGetLastError());
-    }
     // Use `l` lanes per bead from now on (graphs are captured again).
     void set_lpb(int l) {
         if (l == lpb) return;
@@ -274,6 +266,7 @@ struct Part {
         HIPCHK(hipMalloc(&p, n * sizeof(T)));
         HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), stream));
         allocs.push_back(p);
+        alloc_bytes.push_back(n * sizeof(T));
         return (T *)p;
     }
     // Host-to-device copies are synchronous: the source vectors are
@@ -519,8 +512,65 @@ struct Part {
     }
     void release(void *p) {
         auto it = std::find(allocs.begin(), allocs.end(), p);
-        if (it != allocs.end()) allocs.erase(it);
+        if (it != allocs.end()) {
+            alloc_bytes.erase(alloc_bytes.begin() + (it - allocs.begin()));
+            allocs.erase(it);
+        }
         HIPCHK(hipFree(p));
+    }
+    // ------------------------------------------------------ snapshots ---
+    // A copy of every device allocation of the part and of the host-side
+    // step state, so that steps taken only to time them can be undone.  A
+    // snapshot restores only into the allocations it copied (steps that
+    // grow a list reallocate; the tuning steps do not grow lists).
+    struct Snapshot {
+        std::vector<void *> of, copy;
+        int cur = 0;
+        int64_t steps = 0, catchup_steps = 0;
+        std::vector<int> reported, err;
+        std::vector<char> pending_rb;
+    };
+    void take(Snapshot &sn) {
+        HIPCHK(hipStreamSynchronize(stream));
+        sn.of = allocs;
+        sn.copy.assign(allocs.size(), nullptr);
+        for (size_t k = 0; k < allocs.size(); ++k) {
+            HIPCHK(hipMalloc(&sn.copy[k], alloc_bytes[k]));
+            HIPCHK(hipMemcpyAsync(sn.copy[k], allocs[k], alloc_bytes[k], hipMemcpyDeviceToDevice, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        sn.cur = cur;
+        sn.steps = steps;
+        sn.catchup_steps = catchup_steps;
+        sn.reported = reported;
+        sn.pending_rb = pending_rb;
+        sn.err = get(D.err, nrep);
+    }
+    void restore(const Snapshot &sn) {
+        HIPCHK(hipStreamSynchronize(stream));
+        if (sn.of != allocs) throw Fail{XMD_ERR_STATE, "device allocations changed since the snapshot"};
+        for (size_t k = 0; k < allocs.size(); ++k)
+            HIPCHK(hipMemcpyAsync(allocs[k], sn.copy[k], alloc_bytes[k], hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        cur = sn.cur;
+        steps = sn.steps;
+        catchup_steps = sn.catchup_steps;
+        reported = sn.reported;
+        pending_rb = sn.pending_rb;
+        ring_head = ring_n = 0;
+    }
+    static void discard(Snapshot &sn) {
+        for (void *q : sn.copy)
+            if (q) (void)hipFree(q);
+        sn.copy.clear();
+        sn.of.clear();
+    }
+    // True when a replica that was running at the snapshot is halted now.
+    bool halted_since(const Snapshot &sn) {
+        auto e = get(D.err, nrep);
+        for (int k = 0; k < nrep; ++k)
+            if (e[k] && !sn.err[k]) return true;
+        return false;
     }
     void drop_graphs() {
         HIPCHK(hipStreamSynchronize(stream));
@@ -2238,38 +2288,63 @@ xmd_status xmd_autotune(xmd_engine *e, int32_t reps, int32_t *lpb_out, double *m
     static const int cand[3] = {4, 8, 16};
     constexpr int ROUNDS = 3;
     const size_t np = e->parts.size();
-    // every stream group's launches run side by side, as in a step; the
-    // candidates are timed in interleaved rounds and each keeps its median
-    // (the span from the first group's start to the last group's end)
+    // Each candidate times whole graph-captured steps of every stream group
+    // side by side, as xmd_step runs them (the force kernel alone times
+    // differently: with 4 lanes per bead it overlaps a second group's launch
+    // better but the whole step is slower), from the same state: every
+    // part's state is copied first and restored before each sample and at
+    // the end.  Candidates in interleaved rounds, each keeping its median.
+    std::vector<Part::Snapshot> snap(np);
     std::vector<double> t[3];
-    std::vector<hipEvent_t> ev(2 * np, nullptr);
     xmd_status st = XMD_OK;
     for (size_t g = 0; g < np && st == XMD_OK; ++g)
         st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
                          e->parts[g]->ensure_lists();
-                         HIPCHK(hipEventCreate(&ev[2 * g]));
-                         HIPCHK(hipEventCreate(&ev[2 * g + 1]));
+                         e->parts[g]->take(snap[g]);
                      }));
+    // n steps of every part, interleaved, then a wait for all; no list
+    // growth (a capacity error just halts its replica for the sample)
+    auto run = [&](int64_t n) {
+        std::vector<int64_t> left(np, n);
+        bool more = n > 0;
+        while (more && st == XMD_OK) {
+            more = false;
+            for (size_t g = 0; g < np && st == XMD_OK; ++g) {
+                Part *q = e->parts[g];
+                if (left[g] <= 0) continue;
+                st = on_part(e, q, run_guarded(q->err, [&]() { left[g] -= q->launch_next(left[g]); }));
+                more |= left[g] > 0;
+            }
+        }
+        for (size_t g = 0; g < np && st == XMD_OK; ++g)
+            st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() { e->parts[g]->drain(); }));
+    };
+    const int64_t chunk = e->parts[0]->chunk_steps, timed = (int64_t)reps * chunk;
     for (int r = -1; r < ROUNDS && st == XMD_OK; ++r)  // round -1 warms up
         for (int c = 0; c < 3 && st == XMD_OK; ++c) {
             for (size_t g = 0; g < np && st == XMD_OK; ++g)
                 st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
-                                 e->parts[g]->queue_force(cand[c], r < 0 ? 1 : reps, ev[2 * g], ev[2 * g + 1]);
+                                 e->parts[g]->restore(snap[g]);
+                                 e->parts[g]->set_lpb(cand[c]);
                              }));
-            double lo = 0.0, hi = 0.0;
+            run(chunk);  // captures the candidate's graphs (not timed)
+            const auto t0 = std::chrono::steady_clock::now();
+            run(timed);
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            // a sample in which a replica halted stepped fewer replicas
+            bool halted = false;
             for (size_t g = 0; g < np && st == XMD_OK; ++g)
                 st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
-                                 HIPCHK(hipEventSynchronize(ev[2 * g + 1]));
-                                 float ms0 = 0.0f, ms1 = 0.0f;
-                                 HIPCHK(hipEventElapsedTime(&ms0, ev[0], ev[2 * g]));
-                                 HIPCHK(hipEventElapsedTime(&ms1, ev[0], ev[2 * g + 1]));
-                                 lo = g == 0 ? ms0 : std::min(lo, (double)ms0);
-                                 hi = g == 0 ? ms1 : std::max(hi, (double)ms1);
+                                 halted |= e->parts[g]->halted_since(snap[g]);
                              }));
-            if (r >= 0) t[c].push_back((hi - lo) / reps);
+            if (r >= 0) t[c].push_back(halted ? std::numeric_limits<double>::infinity() : ms / (double)timed);
         }
-    for (hipEvent_t x : ev)
-        if (x) (void)hipEventDestroy(x);
+    // every step taken here undone
+    for (size_t g = 0; g < np; ++g) {
+        if (st == XMD_OK)
+            st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() { e->parts[g]->restore(snap[g]); }));
+        Part::discard(snap[g]);
+    }
     if (st != XMD_OK) return st;
     // the fastest tile, unless the default is within 3 % of it: tiles sum
     // a bead's partial forces in different orders, so a choice that follows
