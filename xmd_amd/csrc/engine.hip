@@ -530,9 +530,10 @@ struct Part {
         std::vector<int> reported, err;
         std::vector<char> pending_rb;
     };
+    // (sn.of is set last: a snapshot that failed part-way restores nothing)
     void take(Snapshot &sn) {
         HIPCHK(hipStreamSynchronize(stream));
-        sn.of = allocs;
+        sn.of.clear();
         sn.copy.assign(allocs.size(), nullptr);
         for (size_t k = 0; k < allocs.size(); ++k) {
             HIPCHK(hipMalloc(&sn.copy[k], alloc_bytes[k]));
@@ -545,6 +546,7 @@ struct Part {
         sn.reported = reported;
         sn.pending_rb = pending_rb;
         sn.err = get(D.err, nrep);
+        sn.of = allocs;
     }
     void restore(const Snapshot &sn) {
         HIPCHK(hipStreamSynchronize(stream));
@@ -2288,6 +2290,7 @@ xmd_status xmd_autotune(xmd_engine *e, int32_t reps, int32_t *lpb_out, double *m
     static const int cand[3] = {4, 8, 16};
     constexpr int ROUNDS = 3;
     const size_t np = e->parts.size();
+    const int lpb0 = e->parts[0]->lpb;  // every part runs the same tile
     // Each candidate times whole graph-captured steps of every stream group
     // side by side, as xmd_step runs them (the force kernel alone times
     // differently: with 4 lanes per bead it overlaps a second group's launch
@@ -2339,10 +2342,16 @@ xmd_status xmd_autotune(xmd_engine *e, int32_t reps, int32_t *lpb_out, double *m
                              }));
             if (r >= 0) t[c].push_back(halted ? std::numeric_limits<double>::infinity() : ms / (double)timed);
         }
-    // every step taken here undone
+    // every step taken here undone, also after a failed sample (which
+    // leaves the tile as it was)
     for (size_t g = 0; g < np; ++g) {
-        if (st == XMD_OK)
-            st = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() { e->parts[g]->restore(snap[g]); }));
+        if (!snap[g].of.empty()) {
+            const xmd_status rs = on_part(e, e->parts[g], run_guarded(e->parts[g]->err, [&]() {
+                                              e->parts[g]->restore(snap[g]);
+                                              if (st != XMD_OK) e->parts[g]->set_lpb(lpb0);
+                                          }));
+            if (st == XMD_OK) st = rs;
+        }
         Part::discard(snap[g]);
     }
     if (st != XMD_OK) return st;
