@@ -82,3 +82,32 @@ def test_halted_replica_leaves_the_others_unchanged():
         yb, _, rb, tb = b.get_state(k)
         assert ra == rb and ta == tb, k
         assert np.array_equal(ya, yb), k
+
+
+def test_autotune_with_a_halted_replica_changes_nothing():
+    """The autotuner steps every replica from a copy of the state and
+    restores it: a halted replica stays halted (its error still reported by
+    xmd_replica_status), and every replica's state equals that of a twin
+    engine that did not tune."""
+    system = synth.config_system(3, n_replicas=3)
+    p = synth.config_params(3, precision=_abi.PREC_MIXED)
+    a, b = Engine(p, system), Engine(p, system)
+    for e in (a, b):
+        e.step(10)
+        y1, _, _, t1 = e.get_state(1)
+        e.set_state(1, y1, ZERO_NEXT, t1)
+        with pytest.raises(XmdError):
+            e.step(2)
+    lanes, ms = a.autotune(reps=2)
+    assert lanes in (4, 8, 16) and all(m > 0 for m in ms)
+    assert a.replica_status(1)[0] == _abi.ERR_NONFINITE and a.replica_status(0)[0] == 0
+    for k in range(3):
+        ya, _, ra, ta = a.get_state(k)
+        yb, _, rb, tb = b.get_state(k)
+        assert ra == rb and ta == tb
+        assert np.array_equal(ya, yb, equal_nan=True)
+    if lanes == 8:  # the twins' tiles agree: they step alike from here
+        a.step(5)
+        b.step(5)
+        for k in (0, 2):
+            assert np.array_equal(a.get_state(k)[0], b.get_state(k)[0])
