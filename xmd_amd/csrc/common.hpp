@@ -177,7 +177,7 @@ template <typename T> struct TermTab {
 // Lanes per bead: 4, 8 (default) or 16, chosen by the autotuner.
 constexpr int FORCE_LPB = 8;
 // mutate: the launch is a step's force pass (LJ attractive wall pairs change
-// state); 0 for evaluations and timing launches.
+// state); 0 for evaluations.
 void launch_force(bool mixed, bool eval, int lpb, int nslot, hipStream_t stream, const Dev &D, const Derived &G,
                   const TermTab<float> &Tf, const TermTab<double> &Td, int p, int mutate);
 
