@@ -288,6 +288,8 @@ struct Part {
         return h;
     }
     ~Part() {
+        // nothing queued may still use what is released below
+        if (stream) (void)hipStreamSynchronize(stream);
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         for (auto e : ring)
             if (e) (void)hipEventDestroy(e);
@@ -295,7 +297,6 @@ struct Part {
             if (g) (void)hipGraphExecDestroy(g);
         for (auto &g : g_one)
             if (g) (void)hipGraphExecDestroy(g);
-        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
