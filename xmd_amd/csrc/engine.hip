@@ -336,8 +336,8 @@ struct Part {
     }
     void step_seq(int p) {
         if (G.qa_on) {
-            launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, qa_lds(), stream>>>(D, G, p, nfp_chunks, nct_chunks, 1); });
-            launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, G, p); });
+            launch("k_qa_pairs", [&] { k_qa_pairs<<<qa_grid, BLK, qa_lds(), stream>>>(D, G, p, 1); });
+            launch("k_qa_serial", [&] { k_qa_serial<<<nrep, BLK, 0, stream>>>(D, p); });
         }
         launch("k_force", [&] {
             launch_force(P.precision == XMD_PREC_MIXED, false, lpb, nslot, stream, D, G, TTf, TTd, p, 1);
@@ -1319,8 +1319,6 @@ xmd_status run_guarded(std::string &err, const std::function<void()> &fn) {
 }  // namespace
 
 namespace {
-void *xmd_stream(Part *e) { return e ? (void *)e->stream : nullptr; }
-
 // Device buffer for the duration of one API call (after the stream is
 // drained, so no queued kernel still uses it).
 template <typename T> struct Scratch {
@@ -1507,7 +1505,7 @@ static xmd_status part_eval_forces(Part *e, uint32_t term_mask, double *F, doubl
         uint32_t m = term_mask & e->mask;
         int p = e->cur;
         if (e->G.qa_on && (m & (1u << T_QA)))
-            k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, p, e->nfp_chunks, e->nct_chunks, 0);
+            k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, p, 0);
         int nb = e->nblk();
         TermTab<float> Tf = e->TTf;
         TermTab<double> Td = e->TTd;
@@ -1542,8 +1540,8 @@ static xmd_status part_qa_update(Part *e) {
         e->ensure_lists();
         e->require_running();
         if (!e->G.qa_on) return;
-        k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, e->cur, e->nfp_chunks, e->nct_chunks, 1);
-        k_qa_serial<<<e->nrep, BLK, 0, e->stream>>>(e->D, e->G, e->cur);
+        k_qa_pairs<<<e->qa_grid, BLK, e->qa_lds(), e->stream>>>(e->D, e->G, e->cur, 1);
+        k_qa_serial<<<e->nrep, BLK, 0, e->stream>>>(e->D, e->cur);
         HIPCHK(hipGetLastError());
         e->check_err();
     });

@@ -51,7 +51,7 @@ __device__ inline uint64_t wave_lane_state(const Dev &D, uint64_t S, int w) {
 // Candidates and removals are compacted per chunk in index order, so the
 // serial pass sees them in set order without sorting.  Grid-stride over the
 // live chunks only (those below each replica's live counts).
-__global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int nfp_chunks, int nct_chunks, int mutate) {
+__global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int mutate) {
     // s_pre (2 nrep + 1 ints, dynamic): prefix over the segments "free pairs
     // of replica r" (r < nrep, only when sifting) and "contacts of replica
     // r" of their live chunk counts; block wi of the work sits in segment g
@@ -84,7 +84,6 @@ __global__ __launch_bounds__(BLK) void k_qa_pairs(Dev D, Derived G, int p, int n
     }
     if (tid == 0) s_pre[0] = 0;
     __syncthreads();
-    const int n = D.nslot;
     const double *P = D.pos[p];
     const int ct_tot = D.ct_cap[D.nrep];
     for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
@@ -299,7 +298,7 @@ struct QaBatch {
 // contact set when its extent passes half the capacity.  The workgroup
 // lists the non-empty chunks and stages their entries; the order-dependent
 // steps run on one thread.
-__global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, Derived G, int p) {
+__global__ __launch_bounds__(BLK) void k_qa_serial(Dev D, int p) {
     __shared__ int s_scan[BLK];
     __shared__ int s_list[LIST_SPAN], s_off[LIST_SPAN + 1];
     __shared__ QaBatch s_b;
@@ -1238,7 +1237,6 @@ __global__ __launch_bounds__(BLK) void k_nl_export(Dev D, Derived G, int *cnt, c
     int s = blockIdx.x * BLK + threadIdx.x;
     if (!(D.bflags[s] & B_ACTIVE)) return;
     int rep = D.slot_rep[s];
-    const int n = D.nslot;
     Box box = D.box[rep];
     v3<double> rs = ldp(D.orig_r, s);
     int a = D.rep_slot_off[rep], b = a + D.rep_n[rep];
